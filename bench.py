@@ -1,0 +1,233 @@
+"""Benchmark of the muFFT hot path on one MI355X.
+
+One step = batched 1D FP32 transforms, inputs resident in HBM:
+  c2c forward + c2c inverse, N = 4096, 16384 transforms, and
+  r2c then c2r, N = 65536 real, 512 signals.
+Flop convention of the reference bench (bench.c:468, 500): 5*N*log2(N) per
+c2c transform and 5*N*log2(N) per r2c+c2r pair.
+
+The JSON line carries the roofline of the dominant kernel (the fused
+N = 4096 c2c kernel: algorithmic bytes 16*N per transform over its
+HIP-event-timed launch duration) and a CPU baseline: the reference library
+(oracle/_ref, AVX path, one plan per thread) on a bounded sample of the same
+workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import threading
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "batched c2c FFT GFLOP/s (5N·log2N) + % HBM peak; uneven conv batch speedup"
+
+C2C_N, C2C_B = 4096, 16384
+REAL_N, REAL_B = 65536, 512
+
+
+def flops_per_step() -> float:
+    c2c = 2 * 5.0 * C2C_N * math.log2(C2C_N) * C2C_B
+    real = 5.0 * REAL_N * math.log2(REAL_N) * REAL_B
+    return c2c + real
+
+
+def traffic_from_profiles():
+    """HBM bytes per launch of the c2c kernel from committed rocprofv3 --pmc summaries."""
+    f = ROOT / "profiles" / "traffic_c2c_4096_x16384.json"
+    if not f.exists():
+        return None
+    try:
+        return json.loads(f.read_text()).get("traffic_bytes_per_launch")
+    except Exception:
+        return None
+
+
+def cpu_baseline(x_c2c: np.ndarray, x_real: np.ndarray, budget_s: float = 12.0):
+    """Reference muFFT (AVX path) on host cores, one plan per thread, on a sample."""
+    from oracle import ref
+
+    if not ref.available():
+        return None
+    threads = max(1, min(16, os.cpu_count() or 1))
+    n_c2c = min(len(x_c2c), 64 * threads)
+    n_real = min(len(x_real), 2 * threads)
+    L = ref.lib()
+
+    def work(tid, nrep, out_times):
+        fwd = ref.Plan1D("c2c", C2C_N, -1, 0)
+        inv = ref.Plan1D("c2c", C2C_N, 1, 0)
+        r2c = ref.Plan1D("r2c", REAL_N, -1, 0)
+        c2r = ref.Plan1D("c2r", REAL_N, 1, 0)
+        xs = [ref._copy_in(x_c2c[i], np.complex64) for i in range(tid, n_c2c, threads)]
+        rs = [ref._copy_in(x_real[i], np.float32) for i in range(tid, n_real, threads)]
+        X = fwd.out_like()
+        Y = inv.out_like()
+        S = ref._aligned((REAL_N // 2 + 1) * 8, np.complex64)
+        Z = c2r.out_like()
+        t0 = time.perf_counter()
+        for _ in range(nrep):
+            for a in xs:
+                L.mufft_execute_plan_1d(fwd.h, ref._ptr(X), ref._ptr(a))
+                L.mufft_execute_plan_1d(inv.h, ref._ptr(Y), ref._ptr(X))
+            for a in rs:
+                L.mufft_execute_plan_1d(r2c.h, ref._ptr(S), ref._ptr(a))
+                L.mufft_execute_plan_1d(c2r.h, ref._ptr(Z), ref._ptr(S))
+        out_times[tid] = time.perf_counter() - t0
+
+    # calibrate one repetition, then run a repetition count filling the budget
+    times = [0.0] * threads
+    ths = [threading.Thread(target=work, args=(t, 1, times)) for t in range(threads)]
+    [t.start() for t in ths]
+    [t.join() for t in ths]
+    one = max(times)
+    nrep = max(1, int(budget_s / max(one, 1e-3)))
+    times = [0.0] * threads
+    ths = [threading.Thread(target=work, args=(t, nrep, times)) for t in range(threads)]
+    [t.start() for t in ths]
+    [t.join() for t in ths]
+    wall = max(times)
+    flops = nrep * (n_c2c * 2 * 5.0 * C2C_N * math.log2(C2C_N)
+                    + n_real * 5.0 * REAL_N * math.log2(REAL_N))
+    return {
+        "value": round(flops / wall / 1e9, 3),
+        "unit": "GFLOP/s",
+        "cores": threads,
+        "kind": "reference",
+        "sample": (f"{n_c2c} c2c N={C2C_N} forward+inverse and {n_real} r2c+c2r N={REAL_N} "
+                   f"per repetition, {nrep} repetitions, {threads} threads x one plan each, "
+                   f"muFFT built -O3 with SSE/SSE3/AVX, flags=0 (AVX selected)"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for {args.gpus}); the batch of independent "
+              "transforms would shard across GPUs with no collective.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import mufft_amd as M
+    from oracle import fp64, inputs
+
+    dev = torch.device("cuda:0")
+    t_gen = time.time()
+    x_c2c = inputs.complex_uniform(0x2001, C2C_N * C2C_B).reshape(C2C_B, C2C_N)
+    x_real = inputs.uniform(0x2002, REAL_N * REAL_B).reshape(REAL_B, REAL_N)
+    gen_s = time.time() - t_gen
+
+    dx = torch.from_numpy(x_c2c).to(dev)
+    dX = torch.empty_like(dx)
+    dy = torch.empty_like(dx)
+    dr = torch.from_numpy(x_real).to(dev)
+    dS = torch.empty((REAL_B, REAL_N // 2 + 1), dtype=torch.complex64, device=dev)
+    dz = torch.empty_like(dr)
+
+    fwd = M.Plan1D("c2c", C2C_N, M.FORWARD)
+    inv = M.Plan1D("c2c", C2C_N, M.INVERSE)
+    r2c = M.Plan1D("r2c", REAL_N)
+    c2r = M.Plan1D("c2r", REAL_N)
+    stream = torch.cuda.current_stream(dev)
+
+    ev = []
+
+    def step(record=False):
+        if record:
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+        fwd.execute_batched(dX, dx, C2C_B, stream=stream)
+        if record:
+            e1.record(stream)
+            ev.append((e0, e1))
+        inv.execute_batched(dy, dX, C2C_B, stream=stream)
+        r2c.execute_batched(dS, dr, REAL_B, stream=stream)
+        c2r.execute_batched(dz, dS, REAL_B, stream=stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(record=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = wall * 1e3 / args.steps
+    value = flops_per_step() / (wall / args.steps) / 1e9
+
+    kern_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    alg_bytes = 16.0 * C2C_N * C2C_B
+    achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
+
+    # accuracy on a sample of the measured outputs
+    idx = [0, 1, C2C_B // 2, C2C_B - 1]
+    Xs = dX[idx].cpu().numpy()
+    err_fwd = fp64.rel_rms(Xs, fp64.c2c(x_c2c[idx], -1))
+    err_rt = fp64.rel_rms(dy[idx].cpu().numpy() / C2C_N, x_c2c[idx])
+    ridx = [0, REAL_B - 1]
+    err_r2c = fp64.rel_rms(dS[ridx].cpu().numpy(), fp64.r2c(x_real[ridx]))
+    err_rrt = fp64.rel_rms(dz[ridx].cpu().numpy() / REAL_N, x_real[ridx])
+
+    cpu = None if args.no_cpu else cpu_baseline(x_c2c, x_real)
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (SplitMix64 uniform[-1,1), seeds 0x2001/0x2002)",
+        "config": {
+            "workload": "batched 1D FP32: c2c forward+inverse N=4096 x 16384, r2c then c2r N=65536 x 512",
+            "c2c": {"N": C2C_N, "batch": C2C_B, "directions": ["forward", "inverse"]},
+            "real": {"N": REAL_N, "batch": REAL_B, "sequence": ["r2c", "c2r"]},
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "fft_rows_kernel<4096, forward, c2c>",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic_from_profiles(),
+            "kernel_ms": round(kern_ms, 5),
+            "alg_bytes_per_launch": int(alg_bytes),
+        },
+        "cpu_baseline": cpu,
+        "accuracy": {
+            "c2c_fwd_rel_rms_vs_fp64": err_fwd,
+            "c2c_roundtrip_rel_rms": err_rt,
+            "r2c_rel_rms_vs_fp64": err_r2c,
+            "real_roundtrip_rel_rms": err_rrt,
+        },
+        "conv_batch_speedup": None,
+        "input_gen_s": round(gen_s, 2),
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,72 @@
+/* Extensions of the muFFT API for MI355X (gfx950): batched and stream-ordered
+ * execution on device-resident buffers, device memory helpers and error
+ * reporting.  None of these has a counterpart in the reference (which runs
+ * exactly one transform per call on host memory, fft.h:117, fft.h:272); they
+ * use a separate mufft_hip_ prefix so the reference names stay untouched.
+ *
+ * Conventions
+ *  - Return value: 0 on success, a negative value on error; the message is
+ *    available from mufft_hip_get_last_error() on the calling thread.
+ *  - stream: a hipStream_t passed as void* (NULL = the null stream).
+ *    Batched calls are stream-ordered and asynchronous with respect to the
+ *    host; the drop-in fft.h calls run on a plan-private stream and return
+ *    after the transform has finished.
+ *  - Distances are in elements of the buffer's type (complex float for
+ *    complex data, float for real data); 0 selects the packed layout.
+ */
+#ifndef MUFFT_HIP_EXT_H
+#define MUFFT_HIP_EXT_H
+
+#include <stddef.h>
+
+#include "fft.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error of the last failed call on this thread ("" when none). */
+const char *mufft_hip_get_last_error(void);
+/* Number of visible GPUs (0 when the HIP runtime finds none). */
+int mufft_hip_device_count(void);
+
+/* Device memory and synchronisation helpers. */
+void *mufft_hip_malloc(size_t bytes);
+void mufft_hip_free(void *ptr);
+int mufft_hip_memcpy(void *dst, const void *src, size_t bytes);
+int mufft_hip_synchronize(void *stream);
+
+/* howmany 1D transforms of one plan.  Input element count per transform:
+ * c2c N (N/2 with ZERO_PAD), r2c N reals (N/2 with ZERO_PAD), c2r N/2+1
+ * complex; output: c2c N, r2c N/2+1 complex (N with FULL_R2C), c2r N reals. */
+int mufft_hip_execute_plan_1d_batched(mufft_plan_1d *plan, void *output, const void *input,
+                                      unsigned howmany, size_t idist, size_t odist, void *stream);
+
+/* howmany packed 2D images (layouts of mufft_execute_plan_2d, one after another). */
+int mufft_hip_execute_plan_2d_batched(mufft_plan_2d *plan, void *output, const void *input,
+                                      unsigned howmany, void *stream);
+
+/* Batched convolution halves; block layouts as in mufft_execute_conv_input/output.
+ * Distances between blocks are in complex elements (0 = block size / 8). */
+int mufft_hip_execute_conv_input_batched(mufft_plan_conv *plan, unsigned block, void *output,
+                                         const void *input, unsigned howmany, size_t idist,
+                                         size_t odist, void *stream);
+int mufft_hip_execute_conv_output_batched(mufft_plan_conv *plan, void *output,
+                                          const void *input_first, const void *input_second,
+                                          unsigned howmany, size_t odist, size_t dist_first,
+                                          size_t dist_second, void *stream);
+
+/* Number of kernel launches one execution of the plan issues. */
+int mufft_hip_plan_1d_launches(const mufft_plan_1d *plan);
+int mufft_hip_plan_2d_launches(const mufft_plan_2d *plan);
+
+/* The FP32 Stockham twiddle table the plan of a c2c transform of size N
+ * uses, generated in FP64 on the GPU; copied to host_out (count complex
+ * values, at most the table size).  Returns the table size. */
+int mufft_hip_twiddle_table(unsigned N, int direction, float *host_out, unsigned count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MUFFT_HIP_EXT_H */

@@ -1,0 +1,76 @@
+"""Loads libmufft_hip.so and declares the C ABI (include/mufft/fft.h, mufft_hip.h).
+
+There is no fallback: if the shared library is missing or cannot be loaded,
+importing the package raises, so nothing can silently run on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libmufft_hip.so"
+
+_vp, _u, _i, _f, _sz = ctypes.c_void_p, ctypes.c_uint, ctypes.c_int, ctypes.c_float, ctypes.c_size_t
+
+CONVOLVE_FUNC = ctypes.CFUNCTYPE(None, _vp, _vp, _vp, _f, _u)
+
+# (name, restype, argtypes): every symbol the headers declare.
+SIGNATURES = [
+    # fft.h — reference API
+    ("mufft_create_plan_1d_c2c", _vp, [_u, _i, _u]),
+    ("mufft_create_plan_1d_r2c", _vp, [_u, _u]),
+    ("mufft_create_plan_1d_c2r", _vp, [_u, _u]),
+    ("mufft_execute_plan_1d", None, [_vp, _vp, _vp]),
+    ("mufft_free_plan_1d", None, [_vp]),
+    ("mufft_create_plan_conv", _vp, [_u, _u, _u]),
+    ("mufft_execute_conv_input", None, [_vp, _u, _vp, _vp]),
+    ("mufft_conv_get_transformed_block_size", _sz, [_vp]),
+    ("mufft_execute_conv_output", None, [_vp, _vp, _vp, _vp]),
+    ("mufft_free_plan_conv", None, [_vp]),
+    ("mufft_get_convolve_func", _vp, [_u]),
+    ("mufft_create_plan_2d_c2c", _vp, [_u, _u, _i, _u]),
+    ("mufft_create_plan_2d_r2c", _vp, [_u, _u, _u]),
+    ("mufft_create_plan_2d_c2r", _vp, [_u, _u, _u]),
+    ("mufft_execute_plan_2d", None, [_vp, _vp, _vp]),
+    ("mufft_free_plan_2d", None, [_vp]),
+    ("mufft_alloc", _vp, [_sz]),
+    ("mufft_calloc", _vp, [_sz]),
+    ("mufft_free", None, [_vp]),
+    # mufft_hip.h — extensions
+    ("mufft_hip_get_last_error", ctypes.c_char_p, []),
+    ("mufft_hip_device_count", _i, []),
+    ("mufft_hip_malloc", _vp, [_sz]),
+    ("mufft_hip_free", None, [_vp]),
+    ("mufft_hip_memcpy", _i, [_vp, _vp, _sz]),
+    ("mufft_hip_synchronize", _i, [_vp]),
+    ("mufft_hip_execute_plan_1d_batched", _i, [_vp, _vp, _vp, _u, _sz, _sz, _vp]),
+    ("mufft_hip_execute_plan_2d_batched", _i, [_vp, _vp, _vp, _u, _vp]),
+    ("mufft_hip_execute_conv_input_batched", _i, [_vp, _u, _vp, _vp, _u, _sz, _sz, _vp]),
+    ("mufft_hip_execute_conv_output_batched", _i, [_vp, _vp, _vp, _vp, _u, _sz, _sz, _sz, _vp]),
+    ("mufft_hip_plan_1d_launches", _i, [_vp]),
+    ("mufft_hip_plan_2d_launches", _i, [_vp]),
+    ("mufft_hip_twiddle_table", _i, [_u, _i, _vp, _u]),
+]
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`"
+                " or `python mufft_amd/build.py`")
+        L = ctypes.CDLL(str(LIB_PATH))
+        for name, res, args in SIGNATURES:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    msg = load().mufft_hip_get_last_error()
+    return msg.decode() if msg else ""

@@ -1,0 +1,228 @@
+"""Python mirror of the muFFT plan interface over the C ABI of libmufft_hip.so.
+
+Names, arguments and error behaviour follow the reference API (fft.h):
+creators raise ``MufftError`` where the reference returns NULL (invalid size,
+allocation failure), ``execute`` is synchronous, and the transforms are
+unnormalised.  Buffers can be numpy arrays (host memory, staged by the
+library), torch CUDA tensors (device memory, used in place) or raw integer
+addresses.  ``execute_batched`` runs many transforms of one plan in a single
+stream-ordered call on device memory (library extension).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+FORWARD = -1
+INVERSE = 1
+FLAG_CPU_ANY = 0
+FLAG_CPU_NO_SIMD = (1 << 16) - 1
+FLAG_FULL_R2C = 1 << 16
+FLAG_ZERO_PAD_UPPER_HALF = 1 << 17
+CONV_METHOD_FLAG_MONO_MONO = 0
+CONV_METHOD_FLAG_STEREO_MONO = 1
+CONV_METHOD_FLAG_ZERO_PAD_UPPER_HALF_FIRST = 1 << 2
+CONV_METHOD_FLAG_ZERO_PAD_UPPER_HALF_SECOND = 1 << 3
+CONV_BLOCK_FIRST = 0
+CONV_BLOCK_SECOND = 1
+
+
+class MufftError(RuntimeError):
+    pass
+
+
+def ptr(x) -> int | None:
+    """Address of a buffer: torch tensor, numpy array or int."""
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    if hasattr(x, "data_ptr"):
+        return int(x.data_ptr())
+    if isinstance(x, np.ndarray):
+        if not x.flags.c_contiguous:
+            raise ValueError("buffer must be C-contiguous")
+        return int(x.ctypes.data)
+    raise TypeError(f"unsupported buffer type {type(x)!r}")
+
+
+def stream_handle(s) -> int | None:
+    if s is None:
+        return None
+    if isinstance(s, int):
+        return s
+    h = getattr(s, "cuda_stream", None)
+    if h is not None:
+        return int(h)
+    raise TypeError(f"unsupported stream {type(s)!r}")
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        raise MufftError(f"{what}: {_lib.last_error()}")
+
+
+class Plan1D:
+    """mufft_plan_1d (fft.h:78-121).  kind: 'c2c', 'r2c' or 'c2r'."""
+
+    def __init__(self, kind: str, N: int, direction: int = FORWARD, flags: int = 0):
+        L = _lib.load()
+        self.kind, self.N, self.flags = kind, N, flags
+        self.direction = direction if kind == "c2c" else (FORWARD if kind == "r2c" else INVERSE)
+        if kind == "c2c":
+            h = L.mufft_create_plan_1d_c2c(N, direction, flags)
+        elif kind == "r2c":
+            h = L.mufft_create_plan_1d_r2c(N, flags)
+        elif kind == "c2r":
+            h = L.mufft_create_plan_1d_c2r(N, flags)
+        else:
+            raise ValueError(kind)
+        if not h:
+            raise MufftError(f"mufft_create_plan_1d_{kind}({N}) failed: {_lib.last_error()}")
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def launches(self) -> int:
+        return _lib.load().mufft_hip_plan_1d_launches(self._h)
+
+    def execute(self, out, inp):
+        _lib.load().mufft_execute_plan_1d(self._h, ptr(out), ptr(inp))
+
+    def execute_batched(self, out, inp, howmany: int, idist: int = 0, odist: int = 0, stream=None):
+        rc = _lib.load().mufft_hip_execute_plan_1d_batched(
+            self._h, ptr(out), ptr(inp), howmany, idist, odist, stream_handle(stream))
+        _check(rc, "mufft_hip_execute_plan_1d_batched")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.load().mufft_free_plan_1d(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Plan2D:
+    """mufft_plan_2d (fft.h:221-276).  kind: 'c2c', 'r2c' or 'c2r'."""
+
+    def __init__(self, kind: str, Nx: int, Ny: int, direction: int = FORWARD, flags: int = 0):
+        L = _lib.load()
+        self.kind, self.Nx, self.Ny = kind, Nx, Ny
+        if kind == "c2c":
+            h = L.mufft_create_plan_2d_c2c(Nx, Ny, direction, flags)
+        elif kind == "r2c":
+            h = L.mufft_create_plan_2d_r2c(Nx, Ny, flags)
+        elif kind == "c2r":
+            h = L.mufft_create_plan_2d_c2r(Nx, Ny, flags)
+        else:
+            raise ValueError(kind)
+        if not h:
+            raise MufftError(f"mufft_create_plan_2d_{kind}({Nx}, {Ny}) failed: {_lib.last_error()}")
+        self._h = h
+
+    @property
+    def launches(self) -> int:
+        return _lib.load().mufft_hip_plan_2d_launches(self._h)
+
+    def execute(self, out, inp):
+        _lib.load().mufft_execute_plan_2d(self._h, ptr(out), ptr(inp))
+
+    def execute_batched(self, out, inp, howmany: int, stream=None):
+        rc = _lib.load().mufft_hip_execute_plan_2d_batched(
+            self._h, ptr(out), ptr(inp), howmany, stream_handle(stream))
+        _check(rc, "mufft_hip_execute_plan_2d_batched")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.load().mufft_free_plan_2d(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class PlanConv:
+    """mufft_plan_conv (fft.h:131-207)."""
+
+    def __init__(self, N: int, flags: int = 0, method: int = 0):
+        L = _lib.load()
+        h = L.mufft_create_plan_conv(N, flags, method)
+        if not h:
+            raise MufftError(f"mufft_create_plan_conv({N}) failed: {_lib.last_error()}")
+        self._h, self.N, self.method = h, N, method
+
+    @property
+    def block_size(self) -> int:
+        return int(_lib.load().mufft_conv_get_transformed_block_size(self._h))
+
+    def execute_input(self, block: int, out, inp):
+        _lib.load().mufft_execute_conv_input(self._h, block, ptr(out), ptr(inp))
+
+    def execute_output(self, out, first, second):
+        _lib.load().mufft_execute_conv_output(self._h, ptr(out), ptr(first), ptr(second))
+
+    def execute_input_batched(self, block, out, inp, howmany, idist=0, odist=0, stream=None):
+        rc = _lib.load().mufft_hip_execute_conv_input_batched(
+            self._h, block, ptr(out), ptr(inp), howmany, idist, odist, stream_handle(stream))
+        _check(rc, "mufft_hip_execute_conv_input_batched")
+
+    def execute_output_batched(self, out, first, second, howmany, odist=0, dist_first=0,
+                               dist_second=0, stream=None):
+        rc = _lib.load().mufft_hip_execute_conv_output_batched(
+            self._h, ptr(out), ptr(first), ptr(second), howmany, odist, dist_first, dist_second,
+            stream_handle(stream))
+        _check(rc, "mufft_hip_execute_conv_output_batched")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.load().mufft_free_plan_conv(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def get_convolve_func(flags: int = 0):
+    """mufft_get_convolve_func (fft.h:207) as a Python callable."""
+    addr = _lib.load().mufft_get_convolve_func(flags)
+    if not addr:
+        raise MufftError("mufft_get_convolve_func returned NULL")
+    f = _lib.CONVOLVE_FUNC(addr)
+
+    def call(out, a, b, normalization: float, samples: int):
+        f(ptr(out), ptr(a), ptr(b), ctypes.c_float(normalization), samples)
+
+    return call
+
+
+def twiddle_table(N: int, direction: int) -> np.ndarray:
+    """The FP32 Stockham table of the c2c plan of size N, generated in FP64 on the GPU."""
+    L = _lib.load()
+    size = L.mufft_hip_twiddle_table(N, direction, None, 0)
+    if size < 0:
+        raise MufftError(_lib.last_error())
+    out = np.zeros(max(size, 1), np.complex64)
+    rc = L.mufft_hip_twiddle_table(N, direction, out.ctypes.data, size)
+    if rc < 0:
+        raise MufftError(_lib.last_error())
+    return out[:size]
+
+
+def device_count() -> int:
+    return int(_lib.load().mufft_hip_device_count())

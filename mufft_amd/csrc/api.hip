@@ -1,0 +1,1022 @@
+// Planner, executor and C ABI of the MI355X muFFT drop-in.
+//
+// The reference builds a list of SIMD function pointers per plan and calls
+// them one pass at a time on host memory (fft.c:325-435, fft.c:811-989).
+// Here a plan owns device twiddle tables (generated in FP64 on the GPU), a
+// private HIP stream and scratch space, and executing it enqueues one fused
+// kernel per transform axis whenever the axis fits a workgroup (1D/rows up
+// to 32768 complex points, columns up to 8192), or one generic Stockham pass
+// per launch beyond that.  Host buffers are staged through device memory so
+// the reference's host-memory calling convention keeps working.
+#include <hip/hip_runtime.h>
+
+#include <stdarg.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "kernels.h"
+#include "mufft/fft.h"
+#include "mufft/mufft_hip.h"
+
+using namespace mufft;
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+void set_err(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+}
+
+#define HIP_TRY(expr)                                                              \
+    do {                                                                           \
+        hipError_t e_ = (expr);                                                    \
+        if (e_ != hipSuccess) {                                                    \
+            set_err("%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+            return -1;                                                             \
+        }                                                                          \
+    } while (0)
+
+bool is_pow2(unsigned n) { return n != 0 && (n & (n - 1)) == 0; }
+
+// Grow-only device buffer.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t n) {
+        if (n <= bytes) return 0;
+        release();
+        if (hipMalloc(&p, n) != hipSuccess) {
+            (void)hipGetLastError();
+            set_err("hipMalloc(%zu bytes) failed", n);
+            p = nullptr;
+            return -1;
+        }
+        bytes = n;
+        return 0;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    float2 *f2() const { return static_cast<float2 *>(p); }
+};
+
+// True when kernels cannot dereference the pointer directly (pageable or
+// pinned host memory); such buffers are staged through device memory.
+bool is_host_ptr(const void *ptr) {
+    hipPointerAttribute_t attr;
+    if (hipPointerGetAttributes(&attr, ptr) != hipSuccess) {
+        (void)hipGetLastError();
+        return true;
+    }
+    return !(attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged ||
+             attr.type == hipMemoryTypeArray);
+}
+
+// Stockham twiddle table for a radix sequence, generated on the GPU in FP64.
+int build_stockham_table(int n, int direction, const std::vector<int> &radix, float2 **out,
+                         hipStream_t s) {
+    *out = nullptr;
+    if (radix.size() <= 1) return 0;
+    if (radix.size() > 24) {
+        set_err("radix sequence too long");
+        return -1;
+    }
+    TwiddleArgs a{};
+    a.direction = direction;
+    a.mode = 0;
+    a.r0 = radix[0];
+    a.npass = (int)radix.size();
+    long long p = 1;
+    for (size_t i = 0; i < radix.size(); ++i) {
+        a.radix[i] = radix[i];
+        a.p[i] = p;
+        p *= radix[i];
+    }
+    a.n = n;
+    a.count = n - radix[0];
+    HIP_TRY(hipMalloc(out, sizeof(float2) * (size_t)a.count));
+    a.out = *out;
+    HIP_TRY(launch_twiddles(a, s));
+    return 0;
+}
+
+int build_resolve_table(int n, int direction, float2 **out, hipStream_t s) {
+    TwiddleArgs a{};
+    a.direction = direction;
+    a.mode = 1;
+    a.n = n;
+    a.count = n;
+    HIP_TRY(hipMalloc(out, sizeof(float2) * (size_t)n));
+    a.out = *out;
+    HIP_TRY(launch_twiddles(a, s));
+    return 0;
+}
+
+std::vector<int> fused_sequence(int n) {
+    const int E = row_E(n), NP = row_NP(n);
+    std::vector<int> r;
+    for (int s = 0; s < NP; ++s) r.push_back(seq_radix(n, E, NP, s));
+    return r;
+}
+
+std::vector<int> generic_sequence(int n) {
+    std::vector<int> r;
+    int left = n;
+    while (left >= 16) {
+        r.push_back(16);
+        left /= 16;
+    }
+    if (left > 1) r.push_back(left);
+    return r;
+}
+
+// A complex transform of length n along one axis, fused or generic.
+struct Engine {
+    int n = 0, logn = 0, direction = 0;
+    float2 *tw_fused = nullptr;   // sequence (E, ..., E, r_last) of the fused kernels
+    float2 *tw_gen = nullptr;     // sequence of the generic pass kernels
+    std::vector<int> gen_radix;
+    bool row_fused() const { return logn <= ROW_MAX_LOG2; }
+    bool col_fused() const { return logn <= COL_MAX_LOG2; }
+
+    int init(int n_, int dir, hipStream_t s) {
+        n = n_;
+        logn = ilog2(n_);
+        direction = dir;
+        if (logn <= ROW_MAX_LOG2) {
+            if (build_stockham_table(n, dir, fused_sequence(n), &tw_fused, s)) return -1;
+        }
+        if (logn > COL_MAX_LOG2) {
+            gen_radix = generic_sequence(n);
+            if (build_stockham_table(n, dir, gen_radix, &tw_gen, s)) return -1;
+        }
+        return 0;
+    }
+    void release() {
+        if (tw_fused) (void)hipFree(tw_fused);
+        if (tw_gen) (void)hipFree(tw_gen);
+        tw_fused = tw_gen = nullptr;
+    }
+    int launches_generic() const { return (int)gen_radix.size(); }
+};
+
+// Layout of one operand of a generic pass: transform (img, x) element i at
+// base + img*bs + x + i*sy.
+struct Lay {
+    float2 *p;
+    long long bs, sy;
+};
+
+// Runs the generic pass list from src; the last pass writes `fin`, earlier
+// passes alternate between `fin` and `alt` (src must differ from both).
+int run_generic(const Engine &e, const float2 *src, Lay s_lay, Lay fin, Lay alt, int ncols,
+                int batch, bool zp, hipStream_t st) {
+    const int P = (int)e.gen_radix.size();
+    long long p = 1;
+    const float2 *cur = src;
+    long long cur_bs = s_lay.bs, cur_sy = s_lay.sy;
+    for (int s = 0; s < P; ++s) {
+        const Lay &d = ((P - 1 - s) % 2 == 0) ? fin : alt;
+        PassArgs a{};
+        a.in = cur;
+        a.out = d.p;
+        a.tw = (s == 0) ? nullptr : e.tw_gen + (p - e.gen_radix[0]);
+        a.n = e.n;
+        a.p = p;
+        a.sy_in = cur_sy;
+        a.sy_out = d.sy;
+        a.bs_in = cur_bs;
+        a.bs_out = d.bs;
+        a.ncols = ncols;
+        a.batch = batch;
+        a.flags = (s == 0 && zp) ? RF_ZERO_PAD : 0u;
+        HIP_TRY(launch_pass(e.gen_radix[s], e.direction, a, st));
+        cur = d.p;
+        cur_bs = d.bs;
+        cur_sy = d.sy;
+        p *= e.gen_radix[s];
+    }
+    return 0;
+}
+
+// Contiguous transforms of kind `kind` (rows / 1D): transform g reads
+// in + g*idist (and in2 + g*idist2 with RF_MUL) and writes out + g*odist.
+// Distances in complex elements.  s0/s1 are grown as scratch when needed.
+int run_rows(const Engine &e, int kind, const float2 *rtw, const float2 *in, long long idist,
+             const float2 *in2, long long idist2, float2 *out, long long odist, int batch,
+             unsigned flags, float norm, DevBuf &s0, DevBuf &s1, hipStream_t st) {
+    if (batch <= 0) return 0;
+    if (e.row_fused()) {
+        RowArgs a{};
+        a.in = in;
+        a.in2 = in2;
+        a.out = out;
+        a.tw = e.tw_fused;
+        a.rtw = rtw;
+        a.idist = idist;
+        a.idist2 = idist2;
+        a.odist = odist;
+        a.batch = batch;
+        a.flags = flags;
+        a.norm = norm;
+        HIP_TRY(launch_rows(e.logn, e.direction, kind, a, st));
+        return 0;
+    }
+    // Generic path: packed scratch of batch*n points each.
+    const long long n = e.n;
+    const size_t bytes = sizeof(float2) * (size_t)n * (size_t)batch;
+    if (s0.ensure(bytes) || s1.ensure(bytes)) return -1;
+    const bool zp = (flags & RF_ZERO_PAD) != 0;
+    if (kind == KIND_C2C) {
+        const float2 *src = in;
+        long long sd = idist;
+        bool zp_pass = zp;
+        if (flags & RF_MUL) {
+            MulArgs m{};
+            m.a = in;
+            m.b = in2;
+            m.out = s0.f2();
+            m.n = n;
+            m.adist = idist;
+            m.bdist = idist2;
+            m.odist = n;
+            m.batch = batch;
+            m.norm = norm;
+            m.flags = flags & RF_CONJ2;
+            HIP_TRY(launch_mul(m, st));
+            src = s0.f2();
+            sd = n;
+            zp_pass = false;
+        }
+        return run_generic(e, src, Lay{const_cast<float2 *>(src), sd, 1}, Lay{out, odist, 1},
+                           Lay{s1.f2(), n, 1}, 1, batch, zp_pass, st);
+    }
+    if (kind == KIND_R2C) {
+        if (run_generic(e, in, Lay{const_cast<float2 *>(in), idist, 1}, Lay{s0.f2(), n, 1},
+                        Lay{s1.f2(), n, 1}, 1, batch, zp, st))
+            return -1;
+        ResolveArgs r{};
+        r.in = s0.f2();
+        r.out = out;
+        r.rtw = rtw;
+        r.n = n;
+        r.idist = n;
+        r.odist = odist;
+        r.batch = batch;
+        r.flags = flags & RF_FULL;
+        HIP_TRY(launch_resolve_r2c(r, st));
+        return 0;
+    }
+    // KIND_C2R: resolve (with the optional fused multiply) into s0, then passes.
+    ResolveArgs r{};
+    r.in = in;
+    r.in2 = in2;
+    r.out = s0.f2();
+    r.rtw = rtw;
+    r.n = n;
+    r.idist = idist;
+    r.idist2 = idist2;
+    r.odist = n;
+    r.batch = batch;
+    // With RF_ZERO_PAD the resolve zeroes the upper half of its output: a
+    // zero-padded c2r plan reads only the lower half of the resolved sequence
+    // (radix2_half_p1 after resolve_c2r, fft.c:828-832).
+    r.flags = flags & (RF_MUL | RF_CONJ2 | RF_ZERO_PAD);
+    r.norm = norm;
+    HIP_TRY(launch_resolve_c2r(r, st));
+    return run_generic(e, s0.f2(), Lay{s0.f2(), n, 1}, Lay{out, odist, 1}, Lay{s1.f2(), n, 1}, 1,
+                       batch, false, st);
+}
+
+// Column transforms of length e.n over `ncols` columns, `batch` images.
+// Fused path works in place; the generic path needs src != dst and uses alt.
+int run_cols(const Engine &e, const float2 *src, long long rs_src, long long bs_src, float2 *dst,
+             long long rs_dst, long long bs_dst, int ncols, int batch, DevBuf &alt,
+             hipStream_t st) {
+    if (batch <= 0 || ncols <= 0) return 0;
+    if (e.col_fused()) {
+        ColArgs a{};
+        a.in = src;
+        a.out = dst;
+        a.tw = e.tw_fused;
+        a.in_rs = rs_src;
+        a.out_rs = rs_dst;
+        a.in_bs = bs_src;
+        a.out_bs = bs_dst;
+        a.ncols = ncols;
+        a.ntiles = (ncols + col_TX(e.n) - 1) / col_TX(e.n);
+        a.batch = batch;
+        HIP_TRY(launch_cols(e.logn, e.direction, a, st));
+        return 0;
+    }
+    if (src == dst) {
+        set_err("internal: generic column pass cannot run in place");
+        return -1;
+    }
+    if (alt.ensure(sizeof(float2) * (size_t)bs_dst * (size_t)batch)) return -1;
+    return run_generic(e, src, Lay{const_cast<float2 *>(src), bs_src, rs_src},
+                       Lay{dst, bs_dst, rs_dst}, Lay{alt.f2(), bs_dst, rs_dst}, ncols, batch, false,
+                       st);
+}
+
+// Batched entry points run on the caller's stream; NULL is the null stream.
+hipStream_t resolve_stream(void *s, hipStream_t) { return static_cast<hipStream_t>(s); }
+
+}  // namespace
+
+// ===========================================================================
+// 1D plans
+// ===========================================================================
+struct mufft_plan_1d {
+    unsigned N = 0;      // complex length (reference plan->N, fft.c:58)
+    int direction = 0;
+    int kind = KIND_C2C;
+    unsigned flags = 0;
+    bool full = false, zero_pad = false;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Engine eng;
+    float2 *rtw = nullptr;
+    DevBuf s0, s1, stage_in, stage_in2, stage_out;
+    size_t in_bytes = 0, out_bytes = 0;   // per transform
+    std::mutex lock;                       // serialises drop-in staging
+};
+
+static void destroy_1d(mufft_plan_1d *p) {
+    if (!p) return;
+    if (p->stream) (void)hipStreamSynchronize(p->stream);
+    p->eng.release();
+    if (p->rtw) (void)hipFree(p->rtw);
+    p->s0.release();
+    p->s1.release();
+    p->stage_in.release();
+    p->stage_in2.release();
+    p->stage_out.release();
+    if (p->stream) (void)hipStreamDestroy(p->stream);
+    delete p;
+}
+
+static mufft_plan_1d *create_1d(int kind, unsigned Nc, int direction, unsigned flags) {
+    if (!is_pow2(Nc) || Nc < 2 || Nc > (1u << 27)) {
+        set_err("invalid transform size");
+        return nullptr;
+    }
+    mufft_plan_1d *p = new (std::nothrow) mufft_plan_1d;
+    if (!p) return nullptr;
+    p->N = Nc;
+    p->kind = kind;
+    p->direction = direction;
+    p->flags = flags;
+    p->full = (kind == KIND_R2C) && (flags & MUFFT_FLAG_FULL_R2C);
+    p->zero_pad = (flags & MUFFT_FLAG_ZERO_PAD_UPPER_HALF) != 0;
+    if (hipGetDevice(&p->device) != hipSuccess ||
+        hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipGetLastError();
+        set_err("no usable HIP device");
+        p->stream = nullptr;
+        destroy_1d(p);
+        return nullptr;
+    }
+    if (p->eng.init((int)Nc, direction, p->stream)) {
+        destroy_1d(p);
+        return nullptr;
+    }
+    if (kind != KIND_C2C && build_resolve_table((int)Nc, direction, &p->rtw, p->stream)) {
+        destroy_1d(p);
+        return nullptr;
+    }
+    if (hipStreamSynchronize(p->stream) != hipSuccess) {
+        set_err("twiddle generation failed: %s", hipGetErrorString(hipGetLastError()));
+        destroy_1d(p);
+        return nullptr;
+    }
+    const size_t n = Nc;
+    switch (kind) {
+    case KIND_C2C:
+        p->in_bytes = (p->zero_pad ? n / 2 : n) * 8;
+        p->out_bytes = n * 8;
+        break;
+    case KIND_R2C:
+        p->in_bytes = (p->zero_pad ? n : 2 * n) * 4;
+        p->out_bytes = (p->full ? 2 * n : n + 1) * 8;
+        break;
+    default:
+        p->in_bytes = (n + 1) * 8;
+        p->out_bytes = 2 * n * 4;
+        break;
+    }
+    return p;
+}
+
+// Distances in complex elements; extra = RF_MUL / RF_CONJ2 for fused multiplies.
+static int exec_1d(mufft_plan_1d *p, void *out, const void *in, const void *in2,
+                   unsigned howmany, long long idist, long long idist2, long long odist,
+                   unsigned extra, float norm, hipStream_t st) {
+    unsigned flags = extra;
+    if (p->zero_pad) flags |= RF_ZERO_PAD;
+    if (p->full) flags |= RF_FULL;
+    return run_rows(p->eng, p->kind, p->rtw, static_cast<const float2 *>(in), idist,
+                    static_cast<const float2 *>(in2), idist2, static_cast<float2 *>(out), odist,
+                    (int)howmany, flags, norm, p->s0, p->s1, st);
+}
+
+// Packed distances (complex elements) of a plan's input/output.
+static long long in_dist_c(const mufft_plan_1d *p) {
+    if (p->kind == KIND_C2R) return (long long)p->N + 1;
+    return p->zero_pad ? (long long)p->N / 2 : (long long)p->N;
+}
+static long long out_dist_c(const mufft_plan_1d *p) {
+    if (p->kind == KIND_R2C) return p->full ? 2LL * p->N : (long long)p->N + 1;
+    return p->N;
+}
+
+// Drop-in execution with staging of host buffers; synchronous.
+static void exec_1d_dropin(mufft_plan_1d *p, void *out, const void *in, const void *in2,
+                           size_t in2_bytes, unsigned extra, float norm) {
+    std::lock_guard<std::mutex> g(p->lock);
+    int dev_prev = 0;
+    (void)hipGetDevice(&dev_prev);
+    if (dev_prev != p->device) (void)hipSetDevice(p->device);
+    const void *din = in, *din2 = in2;
+    void *dout = out;
+    const bool hin = is_host_ptr(in), hout = is_host_ptr(out);
+    const bool hin2 = in2 && is_host_ptr(in2);
+    int rc = 0;
+    if (hin) {
+        rc |= p->stage_in.ensure(p->in_bytes);
+        if (!rc) rc |= hipMemcpyAsync(p->stage_in.p, in, p->in_bytes, hipMemcpyHostToDevice, p->stream) != hipSuccess;
+        din = p->stage_in.p;
+    }
+    if (!rc && hin2) {
+        rc |= p->stage_in2.ensure(in2_bytes);
+        if (!rc) rc |= hipMemcpyAsync(p->stage_in2.p, in2, in2_bytes, hipMemcpyHostToDevice, p->stream) != hipSuccess;
+        din2 = p->stage_in2.p;
+    }
+    if (!rc && hout) {
+        rc |= p->stage_out.ensure(p->out_bytes);
+        dout = p->stage_out.p;
+    }
+    if (!rc) rc = exec_1d(p, dout, din, din2, 1, in_dist_c(p), in_dist_c(p), out_dist_c(p), extra, norm, p->stream);
+    if (!rc && hout)
+        rc |= hipMemcpyAsync(out, p->stage_out.p, p->out_bytes, hipMemcpyDeviceToHost, p->stream) != hipSuccess;
+    if (hipStreamSynchronize(p->stream) != hipSuccess || rc) {
+        if (!rc) set_err("execution failed: %s", hipGetErrorString(hipGetLastError()));
+        fprintf(stderr, "mufft_hip: %s\n", g_err);
+    }
+    if (dev_prev != p->device) (void)hipSetDevice(dev_prev);
+}
+
+extern "C" {
+
+mufft_plan_1d *mufft_create_plan_1d_c2c(unsigned N, int direction, unsigned flags) {
+    if (direction != MUFFT_FORWARD && direction != MUFFT_INVERSE) {
+        set_err("invalid direction");
+        return nullptr;
+    }
+    return create_1d(KIND_C2C, N, direction, flags);
+}
+
+mufft_plan_1d *mufft_create_plan_1d_r2c(unsigned N, unsigned flags) {
+    if (!is_pow2(N) || N < 4) {  // N == 2 fails in the reference too (fft.c:485 -> c2c(1))
+        set_err("invalid transform size");
+        return nullptr;
+    }
+    return create_1d(KIND_R2C, N / 2, MUFFT_FORWARD, flags);
+}
+
+mufft_plan_1d *mufft_create_plan_1d_c2r(unsigned N, unsigned flags) {
+    if (!is_pow2(N) || N < 4) {
+        set_err("invalid transform size");
+        return nullptr;
+    }
+    return create_1d(KIND_C2R, N / 2, MUFFT_INVERSE, flags);
+}
+
+void mufft_execute_plan_1d(mufft_plan_1d *plan, void *MUFFT_RESTRICT output,
+                           const void *MUFFT_RESTRICT input) {
+    if (!plan) return;
+    exec_1d_dropin(plan, output, input, nullptr, 0, 0u, 1.0f);
+}
+
+void mufft_free_plan_1d(mufft_plan_1d *plan) { destroy_1d(plan); }
+
+int mufft_hip_execute_plan_1d_batched(mufft_plan_1d *p, void *output, const void *input,
+                                      unsigned howmany, size_t idist, size_t odist, void *stream) {
+    if (!p || !output || !input) {
+        set_err("null argument");
+        return -1;
+    }
+    // Real buffers count floats; two floats make one complex element.
+    long long id, od;
+    if (p->kind == KIND_R2C) {
+        if (idist & 1) {
+            set_err("real input distance must be even");
+            return -1;
+        }
+        id = idist ? (long long)idist / 2 : in_dist_c(p);
+    } else {
+        id = idist ? (long long)idist : in_dist_c(p);
+    }
+    if (p->kind == KIND_C2R) {
+        if (odist & 1) {
+            set_err("real output distance must be even");
+            return -1;
+        }
+        od = odist ? (long long)odist / 2 : (long long)p->N;
+    } else {
+        od = odist ? (long long)odist : out_dist_c(p);
+    }
+    return exec_1d(p, output, input, nullptr, howmany, id, 0, od, 0u, 1.0f,
+                   resolve_stream(stream, p->stream));
+}
+
+int mufft_hip_plan_1d_launches(const mufft_plan_1d *p) {
+    if (!p) return -1;
+    if (p->eng.row_fused()) return 1;
+    return p->eng.launches_generic() + (p->kind == KIND_C2C ? 0 : 1);
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// 2D plans
+// ===========================================================================
+struct mufft_plan_2d {
+    unsigned Nx = 0, Ny = 0;   // Nx = complex columns (Nx_real / 2 for r2c / c2r), fft.c:75
+    int direction = 0;
+    int kind = KIND_C2C;
+    bool full = false, zero_pad = false;
+    unsigned vertical_nx = 0;  // fft.c:85
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Engine ex, ey;
+    float2 *rtw = nullptr;
+    DevBuf s0, s1, s2, alt, stage_in, stage_out;
+    size_t in_bytes = 0, out_bytes = 0;
+    std::mutex lock;
+};
+
+static void destroy_2d(mufft_plan_2d *p) {
+    if (!p) return;
+    if (p->stream) (void)hipStreamSynchronize(p->stream);
+    p->ex.release();
+    p->ey.release();
+    if (p->rtw) (void)hipFree(p->rtw);
+    p->s0.release();
+    p->s1.release();
+    p->s2.release();
+    p->alt.release();
+    p->stage_in.release();
+    p->stage_out.release();
+    if (p->stream) (void)hipStreamDestroy(p->stream);
+    delete p;
+}
+
+static mufft_plan_2d *create_2d(int kind, unsigned Nx_c, unsigned Ny, int direction,
+                                unsigned flags) {
+    if (!is_pow2(Nx_c) || !is_pow2(Ny) || Nx_c < 2 || Ny < 2 || Nx_c > (1u << 20) ||
+        Ny > (1u << 20)) {
+        set_err("invalid transform size");
+        return nullptr;
+    }
+    mufft_plan_2d *p = new (std::nothrow) mufft_plan_2d;
+    if (!p) return nullptr;
+    p->Nx = Nx_c;
+    p->Ny = Ny;
+    p->kind = kind;
+    p->direction = direction;
+    p->full = kind == KIND_R2C && (flags & MUFFT_FLAG_FULL_R2C);
+    p->zero_pad = (flags & MUFFT_FLAG_ZERO_PAD_UPPER_HALF) != 0;
+    if (kind == KIND_C2C) p->vertical_nx = Nx_c;
+    else if (kind == KIND_R2C) p->vertical_nx = p->full ? 2 * Nx_c : Nx_c + 1;
+    else p->vertical_nx = Nx_c + 1;
+    if (hipGetDevice(&p->device) != hipSuccess ||
+        hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipGetLastError();
+        set_err("no usable HIP device");
+        p->stream = nullptr;
+        destroy_2d(p);
+        return nullptr;
+    }
+    if (p->ex.init((int)Nx_c, direction, p->stream) || p->ey.init((int)Ny, direction, p->stream) ||
+        (kind != KIND_C2C && build_resolve_table((int)Nx_c, direction, &p->rtw, p->stream))) {
+        destroy_2d(p);
+        return nullptr;
+    }
+    if (hipStreamSynchronize(p->stream) != hipSuccess) {
+        set_err("twiddle generation failed");
+        destroy_2d(p);
+        return nullptr;
+    }
+    const size_t px = (size_t)Nx_c * Ny;
+    if (kind == KIND_C2C) {
+        p->in_bytes = px * 8;
+        p->out_bytes = px * 8;
+    } else if (kind == KIND_R2C) {
+        p->in_bytes = px * 8;          // Nx_real * Ny floats
+        p->out_bytes = 2 * px * 8;     // Nx_real complex per row
+    } else {
+        p->in_bytes = 2 * px * 8;
+        p->out_bytes = px * 8;         // Nx_real * Ny floats
+    }
+    return p;
+}
+
+// howmany packed images.
+static int exec_2d(mufft_plan_2d *p, void *out_, const void *in_, unsigned howmany,
+                   hipStream_t st) {
+    const long long Nx = p->Nx, Ny = p->Ny;
+    const int hm = (int)howmany;
+    const float2 *in = static_cast<const float2 *>(in_);
+    float2 *out = static_cast<float2 *>(out_);
+    const unsigned zp = p->zero_pad ? RF_ZERO_PAD : 0u;
+    if (p->kind == KIND_C2C || p->kind == KIND_R2C) {
+        const long long rs = p->kind == KIND_C2C ? Nx : 2 * Nx;   // output row stride
+        const long long img = rs * Ny;
+        float2 *rows_dst = out;
+        if (!p->ey.col_fused()) {
+            if (p->s2.ensure(sizeof(float2) * (size_t)img * hm)) return -1;
+            rows_dst = p->s2.f2();
+        }
+        const unsigned f = zp | (p->full ? RF_FULL : 0u);
+        if (run_rows(p->ex, p->kind, p->rtw, in, Nx, nullptr, 0, rows_dst, rs, (int)(Ny * hm), f,
+                     1.0f, p->s0, p->s1, st))
+            return -1;
+        return run_cols(p->ey, rows_dst, rs, img, out, rs, img, (int)p->vertical_nx, hm, p->alt, st);
+    }
+    // c2r: columns first (fft.c:863-916), then the per-row resolve and rows.
+    const long long in_rs = 2 * Nx, in_img = in_rs * Ny;
+    const long long mid_rs = Nx + 1, mid_img = mid_rs * Ny;
+    if (p->s2.ensure(sizeof(float2) * (size_t)mid_img * hm)) return -1;
+    if (run_cols(p->ey, in, in_rs, in_img, p->s2.f2(), mid_rs, mid_img, (int)(Nx + 1), hm, p->alt,
+                 st))
+        return -1;
+    return run_rows(p->ex, KIND_C2R, p->rtw, p->s2.f2(), mid_rs, nullptr, 0, out, Nx,
+                    (int)(Ny * hm), zp, 1.0f, p->s0, p->s1, st);
+}
+
+extern "C" {
+
+mufft_plan_2d *mufft_create_plan_2d_c2c(unsigned Nx, unsigned Ny, int direction, unsigned flags) {
+    if (direction != MUFFT_FORWARD && direction != MUFFT_INVERSE) {
+        set_err("invalid direction");
+        return nullptr;
+    }
+    return create_2d(KIND_C2C, Nx, Ny, direction, flags);
+}
+
+mufft_plan_2d *mufft_create_plan_2d_r2c(unsigned Nx, unsigned Ny, unsigned flags) {
+    if (!is_pow2(Nx) || Nx < 4) {
+        set_err("invalid transform size");
+        return nullptr;
+    }
+    return create_2d(KIND_R2C, Nx / 2, Ny, MUFFT_FORWARD, flags);
+}
+
+mufft_plan_2d *mufft_create_plan_2d_c2r(unsigned Nx, unsigned Ny, unsigned flags) {
+    if (!is_pow2(Nx) || Nx < 4) {
+        set_err("invalid transform size");
+        return nullptr;
+    }
+    return create_2d(KIND_C2R, Nx / 2, Ny, MUFFT_INVERSE, flags);
+}
+
+void mufft_execute_plan_2d(mufft_plan_2d *p, void *MUFFT_RESTRICT output,
+                           const void *MUFFT_RESTRICT input) {
+    if (!p) return;
+    std::lock_guard<std::mutex> g(p->lock);
+    int dev_prev = 0;
+    (void)hipGetDevice(&dev_prev);
+    if (dev_prev != p->device) (void)hipSetDevice(p->device);
+    const void *din = input;
+    void *dout = output;
+    const bool hin = is_host_ptr(input), hout = is_host_ptr(output);
+    int rc = 0;
+    if (hin) {
+        rc |= p->stage_in.ensure(p->in_bytes);
+        if (!rc) rc |= hipMemcpyAsync(p->stage_in.p, input, p->in_bytes, hipMemcpyHostToDevice, p->stream) != hipSuccess;
+        din = p->stage_in.p;
+    }
+    if (!rc && hout) {
+        rc |= p->stage_out.ensure(p->out_bytes);
+        dout = p->stage_out.p;
+        // Columns the transform leaves untouched come back as they went in.
+        if (!rc) rc |= hipMemcpyAsync(p->stage_out.p, output, p->out_bytes, hipMemcpyHostToDevice, p->stream) != hipSuccess;
+    }
+    if (!rc) rc = exec_2d(p, dout, din, 1, p->stream);
+    if (!rc && hout)
+        rc |= hipMemcpyAsync(output, p->stage_out.p, p->out_bytes, hipMemcpyDeviceToHost, p->stream) != hipSuccess;
+    if (hipStreamSynchronize(p->stream) != hipSuccess || rc) {
+        if (!rc) set_err("execution failed: %s", hipGetErrorString(hipGetLastError()));
+        fprintf(stderr, "mufft_hip: %s\n", g_err);
+    }
+    if (dev_prev != p->device) (void)hipSetDevice(dev_prev);
+}
+
+void mufft_free_plan_2d(mufft_plan_2d *plan) { destroy_2d(plan); }
+
+int mufft_hip_execute_plan_2d_batched(mufft_plan_2d *p, void *output, const void *input,
+                                      unsigned howmany, void *stream) {
+    if (!p || !output || !input) {
+        set_err("null argument");
+        return -1;
+    }
+    return exec_2d(p, output, input, howmany, resolve_stream(stream, p->stream));
+}
+
+int mufft_hip_plan_2d_launches(const mufft_plan_2d *p) {
+    if (!p) return -1;
+    int rows = p->ex.row_fused() ? 1 : p->ex.launches_generic() + (p->kind == KIND_C2C ? 0 : 1);
+    int cols = p->ey.col_fused() ? 1 : p->ey.launches_generic();
+    return rows + cols;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Convolution plans (fft.c:565-624, 794-809)
+// ===========================================================================
+struct mufft_plan_conv {
+    unsigned N = 0;
+    unsigned method = 0;
+    bool stereo = false;
+    mufft_plan_1d *plans[2] = {nullptr, nullptr};
+    mufft_plan_1d *output_plan = nullptr;
+    size_t block_size = 0;
+    float normalization = 0.f;
+    unsigned conv_multiply_n = 0;
+};
+
+extern "C" {
+
+void mufft_free_plan_conv(mufft_plan_conv *plan) {
+    if (!plan) return;
+    destroy_1d(plan->plans[0]);
+    destroy_1d(plan->plans[1]);
+    destroy_1d(plan->output_plan);
+    delete plan;
+}
+
+mufft_plan_conv *mufft_create_plan_conv(unsigned N, unsigned flags, unsigned method) {
+    if (!is_pow2(N) || N < 4) {
+        set_err("invalid transform size");
+        return nullptr;
+    }
+    mufft_plan_conv *c = new (std::nothrow) mufft_plan_conv;
+    if (!c) return nullptr;
+    const unsigned ef = (method & MUFFT_CONV_METHOD_FLAG_ZERO_PAD_UPPER_HALF_FIRST) ? MUFFT_FLAG_ZERO_PAD_UPPER_HALF : 0u;
+    const unsigned es = (method & MUFFT_CONV_METHOD_FLAG_ZERO_PAD_UPPER_HALF_SECOND) ? MUFFT_FLAG_ZERO_PAD_UPPER_HALF : 0u;
+    c->N = N;
+    c->method = method;
+    c->stereo = (method & 1) == MUFFT_CONV_METHOD_FLAG_STEREO_MONO;
+    if (!c->stereo) {
+        c->block_size = (N / 2 + 4) * 8;   // fft.c:586 (MUFFT_PADDING_COMPLEX_SAMPLES = 4)
+        c->plans[0] = mufft_create_plan_1d_r2c(N, flags | ef);
+        c->plans[1] = mufft_create_plan_1d_r2c(N, flags | es);
+        c->output_plan = mufft_create_plan_1d_c2r(N, flags);
+        c->conv_multiply_n = N / 2 + 1;
+    } else {
+        c->block_size = (size_t)N * 8;
+        c->plans[0] = mufft_create_plan_1d_c2c(N, MUFFT_FORWARD, flags | ef);
+        c->plans[1] = mufft_create_plan_1d_r2c(N, flags | es | MUFFT_FLAG_FULL_R2C);
+        c->output_plan = mufft_create_plan_1d_c2c(N, MUFFT_INVERSE, flags);
+        c->conv_multiply_n = N;
+    }
+    c->normalization = 1.0f / (float)N;
+    if (!c->plans[0] || !c->plans[1] || !c->output_plan) {
+        mufft_free_plan_conv(c);
+        return nullptr;
+    }
+    return c;
+}
+
+size_t mufft_conv_get_transformed_block_size(mufft_plan_conv *plan) {
+    return plan ? plan->block_size : 0;
+}
+
+void mufft_execute_conv_input(mufft_plan_conv *plan, unsigned block, void *output, const void *input) {
+    if (!plan || block > 1) {
+        set_err("invalid convolution block");
+        return;
+    }
+    mufft_execute_plan_1d(plan->plans[block], output, input);
+}
+
+void mufft_execute_conv_output(mufft_plan_conv *plan, void *output, const void *input_first,
+                               const void *input_second) {
+    if (!plan) return;
+    // The reference multiplies into conv_block and then runs the inverse plan
+    // (fft.c:804-809); here the multiply is the inverse kernel's prologue.
+    mufft_plan_1d *op = plan->output_plan;
+    const size_t nb = (size_t)plan->conv_multiply_n * 8;
+    exec_1d_dropin(op, output, input_first, input_second, nb, RF_MUL, plan->normalization);
+}
+
+int mufft_hip_execute_conv_input_batched(mufft_plan_conv *plan, unsigned block, void *output,
+                                         const void *input, unsigned howmany, size_t idist,
+                                         size_t odist, void *stream) {
+    if (!plan || block > 1) {
+        set_err("invalid convolution block");
+        return -1;
+    }
+    mufft_plan_1d *p = plan->plans[block];
+    const size_t od = odist ? odist : plan->block_size / 8;
+    // Input distances of real inputs are in floats (mufft_hip_execute_plan_1d_batched).
+    return mufft_hip_execute_plan_1d_batched(p, output, input, howmany, idist, od, stream);
+}
+
+int mufft_hip_execute_conv_output_batched(mufft_plan_conv *plan, void *output,
+                                          const void *first, const void *second,
+                                          unsigned howmany, size_t odist, size_t dist_first,
+                                          size_t dist_second, void *stream) {
+    if (!plan || !output || !first || !second) {
+        set_err("null argument");
+        return -1;
+    }
+    mufft_plan_1d *op = plan->output_plan;
+    const long long bd = (long long)(plan->block_size / 8);
+    const long long d1 = dist_first ? (long long)dist_first : bd;
+    const long long d2 = dist_second ? (long long)dist_second : bd;
+    long long od;
+    if (op->kind == KIND_C2R) {
+        if (odist & 1) {
+            set_err("real output distance must be even");
+            return -1;
+        }
+        od = odist ? (long long)odist / 2 : (long long)op->N;
+    } else {
+        od = odist ? (long long)odist : (long long)op->N;
+    }
+    return exec_1d(op, output, first, second, howmany, d1, d2, od, RF_MUL, plan->normalization,
+                   resolve_stream(stream, op->stream));
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Complex multiply function pointer (fft.c:549-563) and memory (fft.c:1032-1084)
+// ===========================================================================
+namespace {
+
+struct MulContext {
+    std::mutex lock;
+    hipStream_t stream = nullptr;
+    DevBuf a, b, o;
+};
+
+MulContext &mul_ctx() {
+    static MulContext ctx;
+    return ctx;
+}
+
+void gpu_convolve(void *output, const void *a, const void *b, float normalization, unsigned samples) {
+    MulContext &c = mul_ctx();
+    std::lock_guard<std::mutex> g(c.lock);
+    if (!c.stream && hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess) {
+        set_err("no usable HIP device");
+        c.stream = nullptr;
+        return;
+    }
+    const size_t bytes = (size_t)samples * 8;
+    const void *da = a, *db = b;
+    void *dout = output;
+    int rc = 0;
+    if (is_host_ptr(a)) {
+        rc |= c.a.ensure(bytes);
+        if (!rc) rc |= hipMemcpyAsync(c.a.p, a, bytes, hipMemcpyHostToDevice, c.stream) != hipSuccess;
+        da = c.a.p;
+    }
+    if (!rc && is_host_ptr(b)) {
+        rc |= c.b.ensure(bytes);
+        if (!rc) rc |= hipMemcpyAsync(c.b.p, b, bytes, hipMemcpyHostToDevice, c.stream) != hipSuccess;
+        db = c.b.p;
+    }
+    const bool hout = is_host_ptr(output);
+    if (!rc && hout) {
+        rc |= c.o.ensure(bytes);
+        dout = c.o.p;
+    }
+    if (!rc) {
+        MulArgs m{};
+        m.a = static_cast<const float2 *>(da);
+        m.b = static_cast<const float2 *>(db);
+        m.out = static_cast<float2 *>(dout);
+        m.n = samples;
+        m.batch = 1;
+        m.norm = normalization;
+        rc |= launch_mul(m, c.stream) != hipSuccess;
+    }
+    if (!rc && hout)
+        rc |= hipMemcpyAsync(output, c.o.p, bytes, hipMemcpyDeviceToHost, c.stream) != hipSuccess;
+    if (hipStreamSynchronize(c.stream) != hipSuccess || rc) {
+        if (!rc) set_err("convolve failed");
+        fprintf(stderr, "mufft_hip: %s\n", g_err);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+mufft_convolve_func mufft_get_convolve_func(unsigned flags) {
+    (void)flags;  // CPU ISA bits have no meaning on the GPU
+    return &gpu_convolve;
+}
+
+void *mufft_alloc(size_t size) {
+    void *p = nullptr;
+    if (hipHostMalloc(&p, size ? size : 1, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        set_err("hipHostMalloc(%zu) failed", size);
+        return nullptr;
+    }
+    return p;
+}
+
+void *mufft_calloc(size_t size) {
+    void *p = mufft_alloc(size);
+    if (p) memset(p, 0, size);
+    return p;
+}
+
+void mufft_free(void *ptr) {
+    if (ptr) (void)hipHostFree(ptr);
+}
+
+const char *mufft_hip_get_last_error(void) { return g_err; }
+
+int mufft_hip_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
+void *mufft_hip_malloc(size_t bytes) {
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) {
+        (void)hipGetLastError();
+        set_err("hipMalloc(%zu) failed", bytes);
+        return nullptr;
+    }
+    return p;
+}
+
+void mufft_hip_free(void *ptr) {
+    if (ptr) (void)hipFree(ptr);
+}
+
+int mufft_hip_memcpy(void *dst, const void *src, size_t bytes) {
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDefault));
+    return 0;
+}
+
+int mufft_hip_synchronize(void *stream) {
+    if (stream) {
+        HIP_TRY(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+    } else {
+        HIP_TRY(hipDeviceSynchronize());
+    }
+    return 0;
+}
+
+int mufft_hip_twiddle_table(unsigned N, int direction, float *host_out, unsigned count) {
+    if (!is_pow2(N) || N < 2 || N > (1u << ROW_MAX_LOG2)) {
+        set_err("invalid transform size");
+        return -1;
+    }
+    hipStream_t s = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    std::vector<int> seq = fused_sequence((int)N);
+    float2 *t = nullptr;
+    int size = seq.size() > 1 ? (int)N - seq[0] : 0;
+    int rc = build_stockham_table((int)N, direction, seq, &t, s);
+    if (!rc && t && host_out && count) {
+        const size_t n = count < (unsigned)size ? count : (unsigned)size;
+        rc = hipMemcpyAsync(host_out, t, n * sizeof(float2), hipMemcpyDeviceToHost, s) != hipSuccess;
+    }
+    rc |= hipStreamSynchronize(s) != hipSuccess;
+    if (t) (void)hipFree(t);
+    (void)hipStreamDestroy(s);
+    if (rc) {
+        set_err("twiddle table generation failed");
+        return -1;
+    }
+    return size;
+}
+
+}  // extern "C"

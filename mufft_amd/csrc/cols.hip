@@ -1,0 +1,88 @@
+// Fused column (strided-axis) transforms: the 2D vertical passes.
+//
+// The reference runs each vertical radix pass as a separate sweep over the
+// whole image (kernel.c:455-738, driven by fft.c:978-987).  Here one
+// workgroup owns a tile of TX adjacent columns and runs every pass of the
+// column transforms in registers/LDS: one HBM read and one HBM write per
+// point, lanes along the unit-stride axis so each row segment of the tile is
+// a single coalesced access.  Tiles are assigned XCD-aware so neighbouring
+// tiles, which share cache lines, run under the same L2.  In place is safe:
+// a workgroup reads its whole tile before it writes any of it.
+#include "kernels.h"
+#include "stockham.h"
+
+namespace mufft {
+
+template <int L_, int DIR>
+__global__ void __launch_bounds__(col_WG(L_)) fft_cols_kernel(ColArgs a) {
+    constexpr int E = col_E(L_);
+    constexpr int LOGE = ilog2(E);
+    constexpr int T = col_T(L_);
+    constexpr int TX = col_TX(L_);
+    constexpr bool SPLIT = col_split(L_);
+    constexpr int ROWS = col_lds_rows(L_);
+    constexpr int SMEM = SPLIT ? (ROWS * TX + 1) / 2 : ROWS * TX;
+    __shared__ float2 smem[SMEM];
+    float *Lf = reinterpret_cast<float *>(smem);
+
+    const int tid = threadIdx.x;
+    const int x = tid & (TX - 1);
+    const int t = tid / TX;
+    const unsigned bid = xcd_block(blockIdx.x, gridDim.x);
+    const int img = (int)(bid / (unsigned)a.ntiles);
+    const int tile = (int)(bid % (unsigned)a.ntiles);
+    const int col = tile * TX + x;
+    const bool active = col < a.ncols && img < a.batch;
+    auto slot = [x](int o) { return pad_slot<LOGE>(o) * TX + x; };
+
+    float2 v[E];
+    const float2 *src = a.in + (long long)img * a.in_bs + col;
+    static_for<0, E>([&](auto C) {
+        constexpr int c = decltype(C)::value;
+        v[c] = active ? src[(long long)(t + T * c) * a.in_rs] : make_float2(0.f, 0.f);
+    });
+
+    stockham_passes<L_, DIR, SPLIT>(v, t, a.tw, smem, Lf, slot);
+
+    if (active) {
+        float2 *dst = a.out + (long long)img * a.out_bs + col;
+        static_for<0, E>([&](auto C) {
+            constexpr int c = decltype(C)::value;
+            dst[(long long)(t + T * c) * a.out_rs] = v[c];
+        });
+    }
+}
+
+template <int DIR>
+static const void *col_kernel_ptr(int logL) {
+    switch (logL) {
+    case 1: return (const void *)&fft_cols_kernel<2, DIR>;
+    case 2: return (const void *)&fft_cols_kernel<4, DIR>;
+    case 3: return (const void *)&fft_cols_kernel<8, DIR>;
+    case 4: return (const void *)&fft_cols_kernel<16, DIR>;
+    case 5: return (const void *)&fft_cols_kernel<32, DIR>;
+    case 6: return (const void *)&fft_cols_kernel<64, DIR>;
+    case 7: return (const void *)&fft_cols_kernel<128, DIR>;
+    case 8: return (const void *)&fft_cols_kernel<256, DIR>;
+    case 9: return (const void *)&fft_cols_kernel<512, DIR>;
+    case 10: return (const void *)&fft_cols_kernel<1024, DIR>;
+    case 11: return (const void *)&fft_cols_kernel<2048, DIR>;
+    case 12: return (const void *)&fft_cols_kernel<4096, DIR>;
+    case 13: return (const void *)&fft_cols_kernel<8192, DIR>;
+    default: return nullptr;
+    }
+}
+
+hipError_t launch_cols(int logL, int direction, const ColArgs &a, hipStream_t s) {
+    const void *k = direction < 0 ? col_kernel_ptr<-1>(logL) : col_kernel_ptr<1>(logL);
+    if (!k) return hipErrorInvalidValue;
+    const int L = 1 << logL;
+    const long long nblocks = (long long)a.ntiles * a.batch;
+    if (nblocks <= 0) return hipSuccess;
+    if (nblocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    ColArgs args = a;
+    void *params[] = {&args};
+    return hipLaunchKernel(k, dim3((unsigned)nblocks), dim3(col_WG(L)), params, 0, s);
+}
+
+}  // namespace mufft

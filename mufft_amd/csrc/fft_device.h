@@ -1,0 +1,102 @@
+// Device-side building blocks: complex arithmetic, constant twiddles and the
+// in-register radix-R DFT used by every gfx950 kernel of this library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <type_traits>
+
+#include "geometry.h"
+
+namespace mufft {
+
+// Compile-time loop: f(std::integral_constant<int, i>) for i in [B, E).
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F &&f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+
+// a * (dir * i)
+template <int DIR>
+__device__ __forceinline__ float2 mul_dir_i(float2 a) {
+    if constexpr (DIR > 0) return make_float2(-a.y, a.x);
+    else return make_float2(a.y, -a.x);
+}
+
+// a * exp(DIR * 2 pi i * J / R) with a compile-time exponent; the eighth-turn
+// cases use the exact structure of the constant instead of a full multiply.
+template <int J, int R, int DIR>
+__device__ __forceinline__ float2 twc(float2 a) {
+    static_assert(R <= 64 && (64 % R) == 0, "radix too large for the constant table");
+    constexpr int j = ((J % R) * (64 / R)) & 63;  // in 64ths of a turn
+    constexpr float h = 0.70710678118654752440f;
+    constexpr float d = DIR > 0 ? 1.0f : -1.0f;
+    if constexpr (j == 0) return a;
+    else if constexpr (j == 16) return mul_dir_i<DIR>(a);
+    else if constexpr (j == 32) return make_float2(-a.x, -a.y);
+    else if constexpr (j == 48) return mul_dir_i<-DIR>(a);
+    else if constexpr (j == 8) return make_float2(h * (a.x - d * a.y), h * (a.y + d * a.x));
+    else if constexpr (j == 24) return make_float2(h * (-a.x - d * a.y), h * (d * a.x - a.y));
+    else if constexpr (j == 40) return make_float2(h * (d * a.y - a.x), h * (-a.y - d * a.x));
+    else if constexpr (j == 56) return make_float2(h * (a.x + d * a.y), h * (a.y - d * a.x));
+    else {
+        constexpr float c = (float)cos64(j);
+        constexpr float s = (float)(d * sin64(j));
+        return make_float2(a.x * c - a.y * s, a.x * s + a.y * c);
+    }
+}
+
+__host__ __device__ constexpr int bitrev(int x, int bits) {
+    return bits == 0 ? 0 : (((x & 1) << (bits - 1)) | bitrev(x >> 1, bits - 1));
+}
+
+// In-register DFT of R points, X[k] = sum_n v[n] exp(DIR * 2 pi i n k / R).
+// Decimation in frequency in place; the bit-reversed result order is undone
+// by a compile-time register renaming (no data movement after unrolling).
+template <int R, int DIR>
+__device__ __forceinline__ void dft(float2 (&v)[R]) {
+    if constexpr (R > 1) {
+        constexpr int L = ilog2(R);
+        static_for<0, L>([&](auto st) {
+            constexpr int span = R >> (decltype(st)::value + 1);
+            static_for<0, R / (2 * span)>([&](auto blk) {
+                static_for<0, span>([&](auto jj) {
+                    constexpr int j = decltype(jj)::value;
+                    constexpr int i0 = decltype(blk)::value * 2 * span + j;
+                    const float2 a = v[i0];
+                    const float2 b = v[i0 + span];
+                    v[i0] = cadd(a, b);
+                    v[i0 + span] = twc<j, 2 * span, DIR>(csub(a, b));
+                });
+            });
+        });
+        float2 t[R];
+        static_for<0, R>([&](auto k) { t[decltype(k)::value] = v[bitrev(decltype(k)::value, L)]; });
+        static_for<0, R>([&](auto k) { v[decltype(k)::value] = t[decltype(k)::value]; });
+    }
+}
+
+// Padded LDS slot of point o when every E points get one spare slot.
+template <int LOGE>
+__device__ __forceinline__ int pad_slot(int o) { return o + (o >> LOGE); }
+
+// blockIdx remap so consecutive logical blocks land on the same XCD (gfx950
+// dispatches workgroups round-robin over its 8 XCDs, each with its own L2):
+// neighbouring column tiles share cache lines, so keeping them on one L2
+// avoids fetching the same line into several L2s.
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) {
+    if ((nb & 7u) != 0u) return b;
+    return (b & 7u) * (nb >> 3) + (b >> 3);
+}
+
+}  // namespace mufft

@@ -1,0 +1,219 @@
+// Shared host/device geometry and constants for the muFFT gfx950 kernels.
+//
+// Both the kernel templates and the host planner derive launch shapes, radix
+// splits, LDS layouts and twiddle-table layouts from the constexpr functions
+// below, so the two sides cannot disagree.
+#pragma once
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define MUFFT_HD __host__ __device__
+#else
+#define MUFFT_HD
+#endif
+
+namespace mufft {
+
+enum Kind : int { KIND_C2C = 0, KIND_R2C = 1, KIND_C2R = 2 };
+
+// Runtime flags of the fused kernels.
+enum RowFlag : unsigned {
+    RF_ZERO_PAD = 1u << 0,   // upper half of the (complex) input is zero, not read
+    RF_FULL = 1u << 1,       // r2c writes all N bins (MUFFT_FLAG_FULL_R2C)
+    RF_MUL = 1u << 2,        // input = norm * in * in2 (fused convolution multiply)
+    RF_CONJ2 = 1u << 3,      // with RF_MUL: use conj(in2) (correlation / adjoint)
+};
+
+MUFFT_HD constexpr int ilog2(long long n) { return n <= 1 ? 0 : 1 + ilog2(n >> 1); }
+
+// ---------------------------------------------------------------------------
+// Fused single-workgroup transform of N complex points (N = 2 .. 32768).
+// Each thread owns E points in registers, T = N/E threads cooperate on one
+// transform, B transforms share a workgroup.  Passes: NP-1 passes of radix E
+// and a last pass of radix N / E^(NP-1).  Between passes the points move
+// through LDS with one padding slot every E points (bank-conflict avoidance);
+// at N >= 32768 the exchange goes through LDS in two halves (real parts, then
+// imaginary parts) so the whole transform fits one workgroup.
+// ---------------------------------------------------------------------------
+MUFFT_HD constexpr int row_E(int N) { return N <= 16 ? N : (N <= 4096 ? 16 : 32); }
+MUFFT_HD constexpr int row_T(int N) { return N / row_E(N); }
+MUFFT_HD constexpr int row_B(int N) { return row_T(N) >= 256 ? 1 : 256 / row_T(N); }
+MUFFT_HD constexpr int row_WG(int N) { return row_T(N) * row_B(N); }
+MUFFT_HD constexpr int row_NP(int N) {
+    return (ilog2(N) + ilog2(row_E(N)) - 1) / ilog2(row_E(N));
+}
+MUFFT_HD constexpr bool row_split(int N) { return N >= 32768; }
+MUFFT_HD constexpr int row_lds_per(int N) { return N + N / row_E(N); }  // padded points
+
+// Radix of pass s for a radix sequence that uses E for all but the last pass.
+MUFFT_HD constexpr int seq_radix(int N, int E, int NP, int s) {
+    return s < NP - 1 ? E : (N >> (ilog2(E) * (NP - 1)));
+}
+MUFFT_HD constexpr int seq_p(int E, int s) { return 1 << (ilog2(E) * s); }
+
+// ---------------------------------------------------------------------------
+// Fused column (strided-axis) transform of length L over a tile of TX
+// adjacent columns.  Lanes run along the columns (unit stride) so every row
+// segment of a tile is one coalesced access.
+// ---------------------------------------------------------------------------
+MUFFT_HD constexpr int col_E(int L) { return row_E(L); }
+MUFFT_HD constexpr int col_T(int L) { return L / col_E(L); }
+MUFFT_HD constexpr int col_TX(int L) {
+    return col_T(L) <= 32 ? 256 / col_T(L) : (col_T(L) <= 128 ? 8 : 4);
+}
+MUFFT_HD constexpr int col_WG(int L) { return col_T(L) * col_TX(L); }
+MUFFT_HD constexpr int col_NP(int L) { return row_NP(L); }
+MUFFT_HD constexpr bool col_split(int L) { return L >= 8192; }
+MUFFT_HD constexpr int col_lds_rows(int L) { return L + L / col_E(L); }
+constexpr int COL_MAX_LOG2 = 13;   // fused column kernel covers L <= 8192
+constexpr int ROW_MAX_LOG2 = 15;   // fused row kernel covers N <= 32768
+
+// Twiddle table layout for a Stockham radix sequence r_0, r_1, ...: pass s >= 1
+// (stride p_s) owns (r_s - 1) * p_s entries starting at p_s - r_0; entry
+// (m - 1) * p_s + k holds exp(dir * 2 pi i * k * m / (r_s * p_s)).  The table
+// has N - r_0 entries in total.
+MUFFT_HD constexpr int tw_offset(int r0, int p) { return p - r0; }
+
+// cos(2 pi j / 64) and sin(2 pi j / 64): constant twiddles of the in-register
+// radix-2..64 butterflies.
+MUFFT_HD constexpr double cos64(int j) {
+    constexpr double t[64] = {
+    1,
+    0.99518472667219693,
+    0.98078528040323043,
+    0.95694033573220882,
+    0.92387953251128674,
+    0.88192126434835505,
+    0.83146961230254524,
+    0.77301045336273699,
+    0.70710678118654757,
+    0.63439328416364549,
+    0.55557023301960229,
+    0.47139673682599781,
+    0.38268343236508984,
+    0.29028467725446233,
+    0.19509032201612833,
+    0.09801714032956077,
+    6.123233995736766e-17,
+    -0.098017140329560645,
+    -0.19509032201612819,
+    -0.29028467725446216,
+    -0.38268343236508973,
+    -0.4713967368259977,
+    -0.55557023301960196,
+    -0.63439328416364538,
+    -0.70710678118654746,
+    -0.77301045336273699,
+    -0.83146961230254535,
+    -0.88192126434835494,
+    -0.92387953251128674,
+    -0.95694033573220882,
+    -0.98078528040323043,
+    -0.99518472667219682,
+    -1,
+    -0.99518472667219693,
+    -0.98078528040323043,
+    -0.95694033573220894,
+    -0.92387953251128685,
+    -0.88192126434835505,
+    -0.83146961230254546,
+    -0.7730104533627371,
+    -0.70710678118654768,
+    -0.63439328416364593,
+    -0.55557023301960218,
+    -0.47139673682599786,
+    -0.38268343236509034,
+    -0.29028467725446244,
+    -0.19509032201612866,
+    -0.098017140329560451,
+    -1.8369701987210297e-16,
+    0.09801714032956009,
+    0.1950903220161283,
+    0.29028467725446205,
+    0.38268343236509,
+    0.47139673682599759,
+    0.55557023301960184,
+    0.6343932841636456,
+    0.70710678118654735,
+    0.77301045336273666,
+    0.83146961230254524,
+    0.88192126434835483,
+    0.92387953251128652,
+    0.95694033573220882,
+    0.98078528040323032,
+    0.99518472667219693
+    };
+    return t[j & 63];
+}
+MUFFT_HD constexpr double sin64(int j) {
+    constexpr double t[64] = {
+    0,
+    0.098017140329560604,
+    0.19509032201612825,
+    0.29028467725446233,
+    0.38268343236508978,
+    0.47139673682599764,
+    0.55557023301960218,
+    0.63439328416364549,
+    0.70710678118654746,
+    0.77301045336273699,
+    0.83146961230254524,
+    0.88192126434835494,
+    0.92387953251128674,
+    0.95694033573220894,
+    0.98078528040323043,
+    0.99518472667219682,
+    1,
+    0.99518472667219693,
+    0.98078528040323043,
+    0.95694033573220894,
+    0.92387953251128674,
+    0.88192126434835505,
+    0.83146961230254546,
+    0.7730104533627371,
+    0.70710678118654757,
+    0.63439328416364549,
+    0.55557023301960218,
+    0.47139673682599786,
+    0.38268343236508989,
+    0.29028467725446239,
+    0.19509032201612861,
+    0.098017140329560826,
+    1.2246467991473532e-16,
+    -0.09801714032956059,
+    -0.19509032201612836,
+    -0.29028467725446211,
+    -0.38268343236508967,
+    -0.47139673682599764,
+    -0.55557023301960196,
+    -0.63439328416364527,
+    -0.70710678118654746,
+    -0.77301045336273666,
+    -0.83146961230254524,
+    -0.88192126434835494,
+    -0.92387953251128652,
+    -0.95694033573220882,
+    -0.98078528040323032,
+    -0.99518472667219693,
+    -1,
+    -0.99518472667219693,
+    -0.98078528040323043,
+    -0.95694033573220894,
+    -0.92387953251128663,
+    -0.88192126434835505,
+    -0.83146961230254546,
+    -0.77301045336273688,
+    -0.70710678118654768,
+    -0.63439328416364593,
+    -0.55557023301960218,
+    -0.47139673682599792,
+    -0.38268343236509039,
+    -0.2902846772544625,
+    -0.19509032201612872,
+    -0.098017140329560506
+    };
+    return t[j & 63];
+}
+
+}  // namespace mufft

@@ -1,0 +1,101 @@
+// Host-visible launch interface of the gfx950 kernels (no HIP device code here).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "geometry.h"
+
+namespace mufft {
+
+// Fused single-workgroup transforms of contiguous sequences (1D, 2D rows).
+struct RowArgs {
+    const float2 *in;    // c2c/r2c: N points per transform (N/2 with zero pad); c2r: N+1 bins
+    const float2 *in2;   // second operand of the fused multiply (RF_MUL)
+    float2 *out;         // c2c: N points; r2c: N+1 bins (2N with RF_FULL); c2r: 2N reals
+    const float2 *tw;    // Stockham twiddles, layout of tw_offset()
+    const float2 *rtw;   // real-transform resolve twiddles W[k], k < N
+    long long idist, idist2, odist;  // distance between transforms, in complex elements
+    int batch;
+    unsigned flags;      // RowFlag
+    float norm;          // scale of the fused multiply
+};
+
+// Fused column transforms of length L over tiles of adjacent columns.
+struct ColArgs {
+    const float2 *in;
+    float2 *out;
+    const float2 *tw;
+    long long in_rs, out_rs;   // row stride (complex elements)
+    long long in_bs, out_bs;   // image stride (complex elements)
+    int ncols;                 // columns to transform (<= row stride)
+    int ntiles;                // column tiles per image
+    int batch;                 // images
+};
+
+// One Stockham pass of radix R over global memory (fallback for any size).
+struct PassArgs {
+    const float2 *in;
+    float2 *out;
+    const float2 *tw;          // this pass's section: entry (m-1)*p + k
+    long long n;               // transform length
+    long long p;               // Stockham stride
+    long long sy_in, sy_out;   // distance between consecutive points of one transform
+    long long bs_in, bs_out;   // distance between transforms
+    int ncols;                 // transforms interleaved with unit stride (columns)
+    int batch;                 // transforms with stride bs
+    unsigned flags;            // RF_ZERO_PAD (first pass only)
+};
+
+struct ResolveArgs {
+    const float2 *in;
+    const float2 *in2;         // RF_MUL second operand (c2r only)
+    float2 *out;
+    const float2 *rtw;
+    long long n;               // complex length N (real length 2N)
+    long long idist, idist2, odist;
+    int batch;
+    unsigned flags;            // RF_FULL (r2c), RF_MUL / RF_CONJ2 (c2r)
+    float norm;
+};
+
+struct MulArgs {
+    const float2 *a, *b;
+    float2 *out;
+    long long n;
+    long long adist, bdist, odist;
+    int batch;
+    float norm;
+    unsigned flags;            // RF_CONJ2
+};
+
+// Twiddle generation in FP64 on the GPU (rounded once to FP32).
+struct TwiddleArgs {
+    float2 *out;
+    int direction;
+    int count;
+    // mode 0: Stockham table for a radix sequence (npass passes with radix[s], p[s]).
+    // mode 1: real-transform resolve twiddles W[k] = (0, dir) * exp(i pi dir k / n).
+    // mode 2: plain table exp(i pi dir k / (n/2)) for k < count (four-step / adjoints).
+    int mode;
+    int r0;
+    int npass;
+    int radix[24];
+    long long p[24];
+    long long n;
+};
+
+hipError_t launch_rows(int logN, int direction, int kind, const RowArgs &a, hipStream_t s);
+hipError_t launch_cols(int logL, int direction, const ColArgs &a, hipStream_t s);
+hipError_t launch_pass(int radix, int direction, const PassArgs &a, hipStream_t s);
+hipError_t launch_resolve_r2c(const ResolveArgs &a, hipStream_t s);
+hipError_t launch_resolve_c2r(const ResolveArgs &a, hipStream_t s);
+hipError_t launch_mul(const MulArgs &a, hipStream_t s);
+hipError_t launch_twiddles(const TwiddleArgs &a, hipStream_t s);
+
+// Geometry queries for the planner.
+inline int rows_threads(int N) { return row_WG(N); }
+inline int rows_per_block(int N) { return row_B(N); }
+
+}  // namespace mufft

@@ -1,0 +1,222 @@
+// Row-kernel dispatch, the generic one-pass-per-launch Stockham kernel (any
+// size, any strides), standalone resolve/multiply kernels and the FP64
+// twiddle generator.
+#include "kernels.h"
+#include "stockham.h"
+
+namespace mufft {
+
+const void *row_kernel_fwd(int logN, int kind);
+const void *row_kernel_inv(int logN, int kind);
+
+hipError_t launch_rows(int logN, int direction, int kind, const RowArgs &a, hipStream_t s) {
+    const void *k = direction < 0 ? row_kernel_fwd(logN, kind) : row_kernel_inv(logN, kind);
+    if (!k) return hipErrorInvalidValue;
+    if (a.batch <= 0) return hipSuccess;
+    const int N = 1 << logN;
+    const long long nblocks = ((long long)a.batch + row_B(N) - 1) / row_B(N);
+    if (nblocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    RowArgs args = a;
+    void *params[] = {&args};
+    return hipLaunchKernel(k, dim3((unsigned)nblocks), dim3(row_WG(N)), params, 0, s);
+}
+
+// ---------------------------------------------------------------------------
+// One Stockham pass over global memory (reference pass semantics,
+// kernel.c:151-168/234-271/395-453 and their vertical forms kernel.c:483-738,
+// generalised to radix 2..16).  Used only where no fused kernel applies.
+// ---------------------------------------------------------------------------
+template <int R, int DIR>
+__global__ void __launch_bounds__(256) fft_pass_kernel(PassArgs a) {
+    const long long nb = a.n / R;
+    const long long total = nb * (long long)a.ncols * a.batch;
+    const bool zp = (a.flags & RF_ZERO_PAD) != 0;
+    for (long long gid = (long long)blockIdx.x * 256 + threadIdx.x; gid < total;
+         gid += (long long)gridDim.x * 256) {
+        const long long x = gid % a.ncols;
+        const long long rest = gid / a.ncols;
+        const long long b = rest % nb;
+        const long long img = rest / nb;
+        const float2 *src = a.in + img * a.bs_in + x;
+        float2 u[R];
+        static_for<0, R>([&](auto M) {
+            constexpr int m = decltype(M)::value;
+            u[m] = (zp && m >= R / 2) ? make_float2(0.f, 0.f) : src[(b + m * nb) * a.sy_in];
+        });
+        const long long k = b & (a.p - 1);
+        if (a.p > 1) {
+            static_for<1, R>([&](auto M) {
+                constexpr int m = decltype(M)::value;
+                u[m] = cmul(u[m], a.tw[(m - 1) * a.p + k]);
+            });
+        }
+        dft<R, DIR>(u);
+        float2 *dst = a.out + img * a.bs_out + x;
+        const long long o = (b - k) * R + k;
+        static_for<0, R>([&](auto Q) {
+            constexpr int q = decltype(Q)::value;
+            dst[(o + q * a.p) * a.sy_out] = u[q];
+        });
+    }
+}
+
+template <int DIR>
+static const void *pass_kernel_ptr(int radix) {
+    switch (radix) {
+    case 2: return (const void *)&fft_pass_kernel<2, DIR>;
+    case 4: return (const void *)&fft_pass_kernel<4, DIR>;
+    case 8: return (const void *)&fft_pass_kernel<8, DIR>;
+    case 16: return (const void *)&fft_pass_kernel<16, DIR>;
+    default: return nullptr;
+    }
+}
+
+static unsigned grid_for(long long work, int per_block) {
+    long long g = (work + per_block - 1) / per_block;
+    if (g < 1) g = 1;
+    if (g > 1048576) g = 1048576;  // grid-stride loops cover the rest
+    return (unsigned)g;
+}
+
+hipError_t launch_pass(int radix, int direction, const PassArgs &a, hipStream_t s) {
+    const void *k = direction < 0 ? pass_kernel_ptr<-1>(radix) : pass_kernel_ptr<1>(radix);
+    if (!k) return hipErrorInvalidValue;
+    const long long work = a.n / radix * (long long)a.ncols * a.batch;
+    if (work <= 0) return hipSuccess;
+    PassArgs args = a;
+    void *params[] = {&args};
+    return hipLaunchKernel(k, dim3(grid_for(work, 256)), dim3(256), params, 0, s);
+}
+
+// ---------------------------------------------------------------------------
+// Standalone real-transform resolves (kernel.c:36-82) for sizes beyond the
+// fused kernels, and the pointwise multiply (kernel.c:22-34).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) resolve_r2c_kernel(ResolveArgs a) {
+    const long long total = a.n * a.batch;
+    const bool full = (a.flags & RF_FULL) != 0;
+    for (long long gid = (long long)blockIdx.x * 256 + threadIdx.x; gid < total;
+         gid += (long long)gridDim.x * 256) {
+        const long long k = gid % a.n;
+        const long long g = gid / a.n;
+        const float2 *Z = a.in + g * a.idist;
+        float2 *X = a.out + g * a.odist;
+        const float2 za = Z[k];
+        if (k == 0) {
+            X[0] = make_float2(za.x + za.y, 0.f);
+            X[a.n] = make_float2(za.x - za.y, 0.f);
+        } else {
+            const float2 zb = cconj(Z[a.n - k]);
+            const float2 fe = cadd(za, zb);
+            const float2 fo = cmul(a.rtw[k], csub(za, zb));
+            X[k] = cscale(cadd(fe, fo), 0.5f);
+            if (full) X[k + a.n] = cscale(csub(fe, fo), 0.5f);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) resolve_c2r_kernel(ResolveArgs a) {
+    const long long total = a.n * a.batch;
+    const bool mul = (a.flags & RF_MUL) != 0;
+    const bool cj = (a.flags & RF_CONJ2) != 0;
+    for (long long gid = (long long)blockIdx.x * 256 + threadIdx.x; gid < total;
+         gid += (long long)gridDim.x * 256) {
+        const long long k = gid % a.n;
+        const long long g = gid / a.n;
+        if ((a.flags & RF_ZERO_PAD) && k >= a.n / 2) {
+            a.out[g * a.odist + k] = make_float2(0.f, 0.f);
+            continue;
+        }
+        const float2 *X = a.in + g * a.idist;
+        float2 xa = X[k];
+        float2 xb = X[a.n - k];
+        if (mul) {
+            const float2 *Y = a.in2 + g * a.idist2;
+            float2 ya = Y[k], yb = Y[a.n - k];
+            if (cj) {
+                ya = cconj(ya);
+                yb = cconj(yb);
+            }
+            xa = cscale(cmul(xa, ya), a.norm);
+            xb = cscale(cmul(xb, yb), a.norm);
+        }
+        const float2 bc = cconj(xb);
+        a.out[g * a.odist + k] = cadd(cadd(xa, bc), cmul(csub(xa, bc), a.rtw[k]));
+    }
+}
+
+__global__ void __launch_bounds__(256) mul_kernel(MulArgs a) {
+    const long long total = a.n * a.batch;
+    const bool cj = (a.flags & RF_CONJ2) != 0;
+    for (long long gid = (long long)blockIdx.x * 256 + threadIdx.x; gid < total;
+         gid += (long long)gridDim.x * 256) {
+        const long long i = gid % a.n;
+        const long long g = gid / a.n;
+        float2 y = a.b[g * a.bdist + i];
+        if (cj) y = cconj(y);
+        a.out[g * a.odist + i] = cscale(cmul(a.a[g * a.adist + i], y), a.norm);
+    }
+}
+
+hipError_t launch_resolve_r2c(const ResolveArgs &a, hipStream_t s) {
+    const long long work = a.n * a.batch;
+    if (work <= 0) return hipSuccess;
+    hipLaunchKernelGGL(resolve_r2c_kernel, dim3(grid_for(work, 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_resolve_c2r(const ResolveArgs &a, hipStream_t s) {
+    const long long work = a.n * a.batch;
+    if (work <= 0) return hipSuccess;
+    hipLaunchKernelGGL(resolve_c2r_kernel, dim3(grid_for(work, 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_mul(const MulArgs &a, hipStream_t s) {
+    const long long work = a.n * a.batch;
+    if (work <= 0) return hipSuccess;
+    hipLaunchKernelGGL(mul_kernel, dim3(grid_for(work, 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Twiddles in FP64, rounded once to FP32.  The phase expression is the
+// reference's twiddle() (fft.c:102-106): (M_PI * direction * k) / p in double.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) twiddle_kernel(TwiddleArgs a) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= a.count) return;
+    const double pi = 3.14159265358979323846;
+    const double dir = (double)a.direction;
+    if (a.mode == 0) {
+        // Locate the pass owning entry e: pass s covers [p_s - r0, p_{s+1} - r0).
+        int s = 1;
+        while (s + 1 < a.npass && (long long)e >= a.p[s + 1] - a.r0) ++s;
+        const long long local = (long long)e - (a.p[s] - a.r0);
+        const long long m = local / a.p[s] + 1;
+        const long long k = local % a.p[s];
+        const long long half = (long long)a.radix[s] * a.p[s] / 2;
+        const double phase = (pi * dir * (double)(k * m)) / (double)half;
+        a.out[e] = make_float2((float)cos(phase), (float)sin(phase));
+    } else if (a.mode == 1) {
+        // fft.c:441-455: W = (0, dir) * twiddle(dir, e, n), complex multiply in float.
+        const double phase = (pi * dir * (double)e) / (double)a.n;
+        const float c = (float)cos(phase);
+        const float sn = (float)sin(phase);
+        const float d = (float)a.direction;
+        const float zero = 0.0f;
+        a.out[e] = make_float2(__fsub_rn(__fmul_rn(zero, c), __fmul_rn(d, sn)),
+                               __fadd_rn(__fmul_rn(zero, sn), __fmul_rn(c, d)));
+    } else {
+        const double phase = (pi * dir * (double)e) / (double)(a.n / 2);
+        a.out[e] = make_float2((float)cos(phase), (float)sin(phase));
+    }
+}
+
+hipError_t launch_twiddles(const TwiddleArgs &a, hipStream_t s) {
+    if (a.count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(twiddle_kernel, dim3((a.count + 255) / 256), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace mufft
