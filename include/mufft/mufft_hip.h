@@ -65,6 +65,24 @@ int mufft_hip_plan_2d_launches(const mufft_plan_2d *plan);
  * values, at most the table size).  Returns the table size. */
 int mufft_hip_twiddle_table(unsigned N, int direction, float *host_out, unsigned count);
 
+/* ---- Fused-kernel variants and the autotuner ----
+ * A size of up to 32768 complex points runs as one fused kernel whose
+ * variant fixes the points per thread, the LDS exchange mode and the twiddle
+ * scheme.  kind: 0 c2c forward, 1 c2c inverse, 2 r2c, 3 c2r; N is the size
+ * passed to the plan creator (real length for r2c / c2r).  A choice applies
+ * to plans created afterwards, process-wide. */
+int mufft_hip_list_variants(unsigned N, int kind, int *out, int max);
+int mufft_hip_default_variant(unsigned N, int kind);
+int mufft_hip_get_variant(unsigned N, int kind);
+/* variant 0 restores the static default. */
+int mufft_hip_set_variant(unsigned N, int kind, int variant);
+/* Times every variant on howmany transforms, checks each against an FP64 FFT
+ * (relative RMS <= 3e-6) and selects the fastest that passes; returns it. */
+int mufft_hip_autotune_1d(unsigned N, int kind, unsigned howmany, double *best_us);
+/* Tune cache as text, one "log2n kind variant us" line per tuned size. */
+int mufft_hip_tune_save(const char *path);
+int mufft_hip_tune_load(const char *path);
+
 #ifdef __cplusplus
 }
 #endif

@@ -50,6 +50,13 @@ SIGNATURES = [
     ("mufft_hip_plan_1d_launches", _i, [_vp]),
     ("mufft_hip_plan_2d_launches", _i, [_vp]),
     ("mufft_hip_twiddle_table", _i, [_u, _i, _vp, _u]),
+    ("mufft_hip_list_variants", _i, [_u, _i, _vp, _i]),
+    ("mufft_hip_default_variant", _i, [_u, _i]),
+    ("mufft_hip_get_variant", _i, [_u, _i]),
+    ("mufft_hip_set_variant", _i, [_u, _i, _i]),
+    ("mufft_hip_autotune_1d", _i, [_u, _i, _u, ctypes.POINTER(ctypes.c_double)]),
+    ("mufft_hip_tune_save", _i, [ctypes.c_char_p]),
+    ("mufft_hip_tune_load", _i, [ctypes.c_char_p]),
 ]
 
 _lib = None

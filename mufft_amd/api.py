@@ -224,5 +224,29 @@ def twiddle_table(N: int, direction: int) -> np.ndarray:
     return out[:size]
 
 
+KIND_CODES = {"c2c_forward": 0, "c2c_inverse": 1, "r2c": 2, "c2r": 3}
+
+
+def autotune(N: int, kind: str, howmany: int = 64) -> tuple[int, float]:
+    """Times every fused-kernel variant of (N, kind) on the device, keeps the
+    fastest one that passes the FP64 check for plans created afterwards, and
+    returns (variant, microseconds)."""
+    L = _lib.load()
+    us = ctypes.c_double(0)
+    v = L.mufft_hip_autotune_1d(N, KIND_CODES[kind], howmany, ctypes.byref(us))
+    if v < 0:
+        raise MufftError(f"autotune({N}, {kind}): {_lib.last_error()}")
+    return v, us.value
+
+
+def set_variant(N: int, kind: str, variant: int) -> None:
+    """Selects a registered variant (0 = static default) for plans created afterwards."""
+    _check(_lib.load().mufft_hip_set_variant(N, KIND_CODES[kind], variant), "mufft_hip_set_variant")
+
+
+def get_variant(N: int, kind: str) -> int:
+    return int(_lib.load().mufft_hip_get_variant(N, KIND_CODES[kind]))
+
+
 def device_count() -> int:
     return int(_lib.load().mufft_hip_device_count())

@@ -16,8 +16,13 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
 #include <mutex>
 #include <new>
+#include <string>
 #include <vector>
 
 #include "kernels.h"
@@ -124,8 +129,10 @@ int build_resolve_table(int n, int direction, float2 **out, hipStream_t s) {
     return 0;
 }
 
-std::vector<int> fused_sequence(int n) {
-    const int E = row_E(n), NP = row_NP(n);
+// Radix sequence (E, ..., E, r_last) of the fused kernels for E points per thread.
+std::vector<int> fused_sequence(int n, int E) {
+    if (E > n) E = n;
+    const int NP = E >= n ? 1 : (ilog2(n) + ilog2(E) - 1) / ilog2(E);
     std::vector<int> r;
     for (int s = 0; s < NP; ++s) r.push_back(seq_radix(n, E, NP, s));
     return r;
@@ -142,32 +149,62 @@ std::vector<int> generic_sequence(int n) {
     return r;
 }
 
-// A complex transform of length n along one axis, fused or generic.
+// ---------------------------------------------------------------------------
+// Process-wide choice of the fused row-kernel variant per (size, transform
+// kind): 0 means the static default rv_default(); the autotuner and
+// mufft_hip_set_variant() fill it.  Plans read it when they are created.
+// ---------------------------------------------------------------------------
+enum TuneKind { TK_C2C_FWD = 0, TK_C2C_INV = 1, TK_R2C = 2, TK_C2R = 3 };
+
+std::mutex g_var_lock;
+int g_variant[ROW_MAX_LOG2 + 1][4] = {};
+
+int tune_kind(int kind, int dir) {
+    if (kind == KIND_C2C) return dir < 0 ? TK_C2C_FWD : TK_C2C_INV;
+    return kind == KIND_R2C ? TK_R2C : TK_C2R;
+}
+
+int chosen_variant(int logn, int tk) {
+    std::lock_guard<std::mutex> g(g_var_lock);
+    const int v = g_variant[logn][tk];
+    return v ? v : rv_default(1 << logn);
+}
+
+// A complex transform of length n along one axis: fused row kernel, fused
+// column kernel or generic passes, each with its own twiddle table.
 struct Engine {
     int n = 0, logn = 0, direction = 0;
-    float2 *tw_fused = nullptr;   // sequence (E, ..., E, r_last) of the fused kernels
+    int row_var = 0;              // fused row-kernel variant
+    float2 *tw_row = nullptr;     // sequence of the row variant
+    float2 *tw_col = nullptr;     // sequence of the column kernel
     float2 *tw_gen = nullptr;     // sequence of the generic pass kernels
     std::vector<int> gen_radix;
     bool row_fused() const { return logn <= ROW_MAX_LOG2; }
     bool col_fused() const { return logn <= COL_MAX_LOG2; }
 
-    int init(int n_, int dir, hipStream_t s) {
+    // rows / cols: which roles this engine serves; kind: KIND_* of the row role.
+    int init(int n_, int dir, int kind, bool rows, bool cols, hipStream_t s, int force_var = 0) {
         n = n_;
         logn = ilog2(n_);
         direction = dir;
-        if (logn <= ROW_MAX_LOG2) {
-            if (build_stockham_table(n, dir, fused_sequence(n), &tw_fused, s)) return -1;
+        if (rows && row_fused()) {
+            row_var = force_var ? force_var : chosen_variant(logn, tune_kind(kind, dir));
+            if (build_stockham_table(n, dir, fused_sequence(n, rv_E(n, row_var)), &tw_row, s)) return -1;
         }
-        if (logn > COL_MAX_LOG2) {
+        if (cols && col_fused()) {
+            if (build_stockham_table(n, dir, fused_sequence(n, col_E(n)), &tw_col, s)) return -1;
+        }
+        if ((rows && !row_fused()) || (cols && !col_fused())) {
             gen_radix = generic_sequence(n);
             if (build_stockham_table(n, dir, gen_radix, &tw_gen, s)) return -1;
         }
         return 0;
     }
     void release() {
-        if (tw_fused) (void)hipFree(tw_fused);
+        if (tw_row) (void)hipFree(tw_row);
+        if (tw_col) (void)hipFree(tw_col);
         if (tw_gen) (void)hipFree(tw_gen);
-        tw_fused = tw_gen = nullptr;
+        tw_row = tw_col = tw_gen = nullptr;
     }
     int launches_generic() const { return (int)gen_radix.size(); }
 };
@@ -223,7 +260,7 @@ int run_rows(const Engine &e, int kind, const float2 *rtw, const float2 *in, lon
         a.in = in;
         a.in2 = in2;
         a.out = out;
-        a.tw = e.tw_fused;
+        a.tw = e.tw_row;
         a.rtw = rtw;
         a.idist = idist;
         a.idist2 = idist2;
@@ -231,7 +268,7 @@ int run_rows(const Engine &e, int kind, const float2 *rtw, const float2 *in, lon
         a.batch = batch;
         a.flags = flags;
         a.norm = norm;
-        HIP_TRY(launch_rows(e.logn, e.direction, kind, a, st));
+        HIP_TRY(launch_rows(e.logn, e.direction, kind, e.row_var, a, st));
         return 0;
     }
     // Generic path: packed scratch of batch*n points each.
@@ -310,7 +347,7 @@ int run_cols(const Engine &e, const float2 *src, long long rs_src, long long bs_
         ColArgs a{};
         a.in = src;
         a.out = dst;
-        a.tw = e.tw_fused;
+        a.tw = e.tw_col;
         a.in_rs = rs_src;
         a.out_rs = rs_dst;
         a.in_bs = bs_src;
@@ -389,7 +426,7 @@ static mufft_plan_1d *create_1d(int kind, unsigned Nc, int direction, unsigned f
         destroy_1d(p);
         return nullptr;
     }
-    if (p->eng.init((int)Nc, direction, p->stream)) {
+    if (p->eng.init((int)Nc, direction, kind, true, false, p->stream)) {
         destroy_1d(p);
         return nullptr;
     }
@@ -610,7 +647,8 @@ static mufft_plan_2d *create_2d(int kind, unsigned Nx_c, unsigned Ny, int direct
         destroy_2d(p);
         return nullptr;
     }
-    if (p->ex.init((int)Nx_c, direction, p->stream) || p->ey.init((int)Ny, direction, p->stream) ||
+    if (p->ex.init((int)Nx_c, direction, kind, true, false, p->stream) ||
+        p->ey.init((int)Ny, direction, KIND_C2C, false, true, p->stream) ||
         (kind != KIND_C2C && build_resolve_table((int)Nx_c, direction, &p->rtw, p->stream))) {
         destroy_2d(p);
         return nullptr;
@@ -1001,7 +1039,7 @@ int mufft_hip_twiddle_table(unsigned N, int direction, float *host_out, unsigned
     }
     hipStream_t s = nullptr;
     HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    std::vector<int> seq = fused_sequence((int)N);
+    std::vector<int> seq = fused_sequence((int)N, rv_E((int)N, rv_default((int)N)));
     float2 *t = nullptr;
     int size = seq.size() > 1 ? (int)N - seq[0] : 0;
     int rc = build_stockham_table((int)N, direction, seq, &t, s);
@@ -1017,6 +1055,285 @@ int mufft_hip_twiddle_table(unsigned N, int direction, float *host_out, unsigned
         return -1;
     }
     return size;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Autotuner: times every registered fused-kernel variant of a size and keeps
+// the fastest one whose output passes an FP64 check.  The static choice
+// rv_default() stays in force until a size is tuned or set explicitly.
+// ===========================================================================
+namespace {
+
+using cd = std::complex<double>;
+
+// Iterative radix-2 FFT in FP64, X[k] = sum x[n] exp(dir 2 pi i n k / n).
+void fft64(std::vector<cd> &a, int dir) {
+    const size_t n = a.size();
+    for (size_t i = 1, j = 0; i < n; ++i) {
+        size_t bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) std::swap(a[i], a[j]);
+    }
+    for (size_t len = 2; len <= n; len <<= 1) {
+        const double ang = dir * 2.0 * M_PI / (double)len;
+        for (size_t i = 0; i < n; i += len)
+            for (size_t k = 0; k < len / 2; ++k) {
+                const cd w(std::cos(ang * (double)k), std::sin(ang * (double)k));
+                const cd u = a[i + k], v = a[i + k + len / 2] * w;
+                a[i + k] = u + v;
+                a[i + k + len / 2] = u - v;
+            }
+    }
+}
+
+// FP64 reference of one transform of kind tk on complex length nc, using the
+// reference semantics (r2c N/2+1 bins; c2r keeps Im of DC and Nyquist).
+std::vector<cd> reference64(int tk, int nc, const float2 *in) {
+    if (tk == TK_C2C_FWD || tk == TK_C2C_INV) {
+        std::vector<cd> a(nc);
+        for (int i = 0; i < nc; ++i) a[i] = cd(in[i].x, in[i].y);
+        fft64(a, tk == TK_C2C_FWD ? -1 : 1);
+        return a;
+    }
+    const int m = 2 * nc;
+    if (tk == TK_R2C) {
+        const float *x = reinterpret_cast<const float *>(in);
+        std::vector<cd> a(m);
+        for (int i = 0; i < m; ++i) a[i] = cd(x[i], 0.0);
+        fft64(a, -1);
+        a.resize(nc + 1);
+        return a;
+    }
+    std::vector<cd> f(m);
+    f[0] = cd(in[0].x, 0.0);
+    f[nc] = cd(in[nc].x, 0.0);
+    for (int k = 1; k < nc; ++k) {
+        f[k] = cd(in[k].x, in[k].y);
+        f[m - k] = std::conj(f[k]);
+    }
+    fft64(f, 1);
+    std::vector<cd> y(nc);  // output reals packed as complex pairs
+    for (int i = 0; i < m; ++i) {
+        const double q2 = -(double)in[0].y * ((i & 1) ? -1.0 : 1.0) - (double)in[nc].y;
+        const double val = f[i].real() + q2;
+        if (i & 1) y[i / 2].imag(val);
+        else y[i / 2].real(val);
+    }
+    return y;
+}
+
+struct TuneRecord {
+    int logn, tk, variant;
+    double us;
+};
+std::vector<TuneRecord> g_tune_records;
+
+int tk_from_api(int kind) { return (kind >= 0 && kind <= 3) ? kind : -1; }
+
+// Complex log2 length of an API size (real sizes for r2c / c2r).
+int logn_from_api(unsigned N, int tk) {
+    if (!is_pow2(N)) return -1;
+    const int l = ilog2(N) - ((tk == TK_R2C || tk == TK_C2R) ? 1 : 0);
+    return (l >= 1 && l <= ROW_MAX_LOG2) ? l : -1;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mufft_hip_list_variants(unsigned N, int kind, int *out, int max) {
+    const int tk = tk_from_api(kind);
+    const int l = tk < 0 ? -1 : logn_from_api(N, tk);
+    if (l < 0) {
+        set_err("no fused kernel for this size");
+        return -1;
+    }
+    return row_variant_list(l, out, max);
+}
+
+int mufft_hip_default_variant(unsigned N, int kind) {
+    const int tk = tk_from_api(kind);
+    const int l = tk < 0 ? -1 : logn_from_api(N, tk);
+    if (l < 0) {
+        set_err("no fused kernel for this size");
+        return -1;
+    }
+    return rv_default(1 << l);
+}
+
+int mufft_hip_get_variant(unsigned N, int kind) {
+    const int tk = tk_from_api(kind);
+    const int l = tk < 0 ? -1 : logn_from_api(N, tk);
+    if (l < 0) {
+        set_err("no fused kernel for this size");
+        return -1;
+    }
+    return chosen_variant(l, tk);
+}
+
+int mufft_hip_set_variant(unsigned N, int kind, int variant) {
+    const int tk = tk_from_api(kind);
+    const int l = tk < 0 ? -1 : logn_from_api(N, tk);
+    if (l < 0 || (variant != 0 && !row_variant_registered(l, variant))) {
+        set_err("unknown size or variant");
+        return -1;
+    }
+    std::lock_guard<std::mutex> g(g_var_lock);
+    g_variant[l][tk] = variant;
+    return 0;
+}
+
+int mufft_hip_autotune_1d(unsigned N, int kind, unsigned howmany, double *best_us) {
+    const int tk = tk_from_api(kind);
+    const int l = tk < 0 ? -1 : logn_from_api(N, tk);
+    if (l < 0 || howmany == 0) {
+        set_err("no fused kernel for this size");
+        return -1;
+    }
+    const int nc = 1 << l;
+    const int fkind = tk == TK_R2C ? KIND_R2C : (tk == TK_C2R ? KIND_C2R : KIND_C2C);
+    const int dir = (tk == TK_C2C_FWD || tk == TK_R2C) ? -1 : 1;
+    const long long idist = tk == TK_C2R ? nc + 1 : nc;
+    const long long odist = tk == TK_R2C ? nc + 1 : nc;
+    std::vector<float2> h_in((size_t)idist * howmany);
+    uint64_t st = 0x9E3779B97F4A7C15ull ^ ((uint64_t)nc << 8) ^ (uint64_t)tk;
+    for (auto &z : h_in) {
+        st = st * 6364136223846793005ull + 1442695040888963407ull;
+        z.x = (float)((double)(st >> 40) / 16777216.0 - 0.5);
+        st = st * 6364136223846793005ull + 1442695040888963407ull;
+        z.y = (float)((double)(st >> 40) / 16777216.0 - 0.5);
+    }
+    const int nval = (int)std::min<unsigned>(howmany, 2);
+    std::vector<std::vector<cd>> ref(nval);
+    for (int b = 0; b < nval; ++b) ref[b] = reference64(tk, nc, h_in.data() + (size_t)b * idist);
+
+    hipStream_t s = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    DevBuf din, dout, s0, s1;
+    float2 *rtw = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int best = -1;
+    double best_t = 1e30;
+    int rc = 0;
+    if (din.ensure(h_in.size() * sizeof(float2)) || dout.ensure((size_t)odist * howmany * sizeof(float2)) ||
+        hipMemcpy(din.p, h_in.data(), h_in.size() * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess ||
+        (fkind != KIND_C2C && build_resolve_table(nc, dir, &rtw, s)) ||
+        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        rc = -1;
+    }
+    int vars[64];
+    const int nv = rc ? 0 : std::min(row_variant_list(l, vars, 64), 64);
+    std::vector<float2> h_out((size_t)odist * nval);
+    for (int i = 0; i < nv && !rc; ++i) {
+        Engine e;
+        if (e.init(nc, dir, fkind, true, false, s, vars[i])) {
+            e.release();
+            continue;
+        }
+        RowArgs a{};
+        a.in = din.f2();
+        a.out = dout.f2();
+        a.tw = e.tw_row;
+        a.rtw = rtw;
+        a.idist = idist;
+        a.odist = odist;
+        a.batch = (int)howmany;
+        bool ok = launch_rows(l, dir, fkind, vars[i], a, s) == hipSuccess &&
+                  hipMemcpyAsync(h_out.data(), dout.p, h_out.size() * sizeof(float2),
+                                 hipMemcpyDeviceToHost, s) == hipSuccess &&
+                  hipStreamSynchronize(s) == hipSuccess;
+        double err = 0;
+        for (int b = 0; ok && b < nval; ++b) {
+            double num = 0, den = 0;
+            const size_t cnt = ref[b].size();
+            for (size_t k = 0; k < cnt; ++k) {
+                const float2 g = h_out[(size_t)b * odist + k];
+                num += std::norm(cd(g.x, g.y) - ref[b][k]);
+                den += std::norm(ref[b][k]);
+            }
+            err = std::max(err, std::sqrt(num / (den > 0 ? den : 1.0)));
+        }
+        ok = ok && err <= 3e-6;
+        if (ok) {
+            const int reps = 5;
+            (void)launch_rows(l, dir, fkind, vars[i], a, s);
+            (void)hipEventRecord(e0, s);
+            for (int r = 0; r < reps; ++r) (void)launch_rows(l, dir, fkind, vars[i], a, s);
+            (void)hipEventRecord(e1, s);
+            float ms = 0;
+            ok = hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+            const double us = 1e3 * ms / reps;
+            if (ok && us < best_t) {
+                best_t = us;
+                best = vars[i];
+            }
+        }
+        (void)hipStreamSynchronize(s);
+        e.release();
+    }
+    if (rtw) (void)hipFree(rtw);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(s);
+    if (rc || best < 0) {
+        if (!rc) set_err("no variant passed validation");
+        return -1;
+    }
+    {
+        std::lock_guard<std::mutex> g(g_var_lock);
+        g_variant[l][tk] = best;
+        g_tune_records.erase(std::remove_if(g_tune_records.begin(), g_tune_records.end(),
+                                            [&](const TuneRecord &t) { return t.logn == l && t.tk == tk; }),
+                             g_tune_records.end());
+        g_tune_records.push_back({l, tk, best, best_t});
+    }
+    if (best_us) *best_us = best_t;
+    return best;
+}
+
+// Tune cache: one line per entry, "log2n kind variant microseconds".
+int mufft_hip_tune_save(const char *path) {
+    FILE *f = fopen(path, "w");
+    if (!f) {
+        set_err("cannot write %s", path);
+        return -1;
+    }
+    std::lock_guard<std::mutex> g(g_var_lock);
+    fprintf(f, "# mufft_hip tune cache: log2(complex length) kind(0 c2c fwd,1 c2c inv,2 r2c,3 c2r) variant us\n");
+    for (int l = 0; l <= ROW_MAX_LOG2; ++l)
+        for (int tk = 0; tk < 4; ++tk)
+            if (g_variant[l][tk]) {
+                double us = 0;
+                for (const auto &t : g_tune_records)
+                    if (t.logn == l && t.tk == tk) us = t.us;
+                fprintf(f, "%d %d %d %.3f\n", l, tk, g_variant[l][tk], us);
+            }
+    fclose(f);
+    return 0;
+}
+
+int mufft_hip_tune_load(const char *path) {
+    FILE *f = fopen(path, "r");
+    if (!f) {
+        set_err("cannot read %s", path);
+        return -1;
+    }
+    char line[256];
+    int n = 0;
+    while (fgets(line, sizeof line, f)) {
+        int l, tk, var;
+        double us;
+        if (line[0] == '#' || sscanf(line, "%d %d %d %lf", &l, &tk, &var, &us) != 4) continue;
+        if (l < 1 || l > ROW_MAX_LOG2 || tk < 0 || tk > 3 || !row_variant_registered(l, var)) continue;
+        std::lock_guard<std::mutex> g(g_var_lock);
+        g_variant[l][tk] = var;
+        ++n;
+    }
+    fclose(f);
+    return n;
 }
 
 }  // extern "C"

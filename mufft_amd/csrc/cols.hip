@@ -19,11 +19,8 @@ __global__ void __launch_bounds__(col_WG(L_)) fft_cols_kernel(ColArgs a) {
     constexpr int LOGE = ilog2(E);
     constexpr int T = col_T(L_);
     constexpr int TX = col_TX(L_);
-    constexpr bool SPLIT = col_split(L_);
     constexpr int ROWS = col_lds_rows(L_);
-    constexpr int SMEM = SPLIT ? (ROWS * TX + 1) / 2 : ROWS * TX;
-    __shared__ float2 smem[SMEM];
-    float *Lf = reinterpret_cast<float *>(smem);
+    __shared__ float2 smem[ROWS * TX];
 
     const int tid = threadIdx.x;
     const int x = tid & (TX - 1);
@@ -42,7 +39,7 @@ __global__ void __launch_bounds__(col_WG(L_)) fft_cols_kernel(ColArgs a) {
         v[c] = active ? src[(long long)(t + T * c) * a.in_rs] : make_float2(0.f, 0.f);
     });
 
-    stockham_passes<L_, DIR, SPLIT>(v, t, a.tw, smem, Lf, slot);
+    stockham_passes<L_, E, DIR, XM_PAD, TW_FULL>(v, t, TwGlobal{a.tw}, smem, nullptr, slot);
 
     if (active) {
         float2 *dst = a.out + (long long)img * a.out_bs + col;

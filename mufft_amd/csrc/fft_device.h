@@ -86,6 +86,36 @@ __device__ __forceinline__ void dft(float2 (&v)[R]) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Buffer-resource access.  A strided access base + lt + T*c becomes one VGPR
+// offset (lt) plus a scalar offset (T*c) per slot, instead of one 64-bit
+// address per slot; the descriptor's byte limit also bounds every access
+// (out-of-range loads read 0, out-of-range stores are dropped).
+// ---------------------------------------------------------------------------
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+// Word 3 of a raw (untyped) gfx9 buffer descriptor: 32-bit data format.
+constexpr int kBufWord3 = 0x00020000;
+
+// The base pointer and size must be uniform across the wave.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *p, unsigned bytes) {
+    const unsigned long long u = reinterpret_cast<unsigned long long>(p);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
+    void *q = reinterpret_cast<void *>(((unsigned long long)hi << 32) | lo);
+    return __builtin_amdgcn_make_buffer_rsrc(q, (short)0, (int)__builtin_amdgcn_readfirstlane(bytes),
+                                             kBufWord3);
+}
+
+__device__ __forceinline__ float2 bload(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+}
+
+__device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff,
+                                       float2 v) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, (int)voff, (int)soff, 0);
+}
+
 // Padded LDS slot of point o when every E points get one spare slot.
 template <int LOGE>
 __device__ __forceinline__ int pad_slot(int o) { return o + (o >> LOGE); }

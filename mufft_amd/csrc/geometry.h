@@ -46,6 +46,42 @@ MUFFT_HD constexpr int row_NP(int N) {
 MUFFT_HD constexpr bool row_split(int N) { return N >= 32768; }
 MUFFT_HD constexpr int row_lds_per(int N) { return N + N / row_E(N); }  // padded points
 
+// ---------------------------------------------------------------------------
+// Variants of the fused row kernel.  A variant fixes E (points per thread,
+// clamped to N), the LDS exchange mode (XMode in stockham.h: 0 padded complex,
+// 1 split real/imaginary, 2 unpadded complex) and the twiddle mode (TwMode:
+// 0 every twiddle from the table, 1 split product).  The planner's static
+// choice is rv_default(); the autotuner may pick any registered variant.
+// ---------------------------------------------------------------------------
+MUFFT_HD constexpr int rv_make(int E, int xm, int twm) { return E | (xm << 8) | (twm << 12); }
+MUFFT_HD constexpr int rv_E(int N, int var) { return (var & 0xff) >= N ? N : (var & 0xff); }
+MUFFT_HD constexpr int rv_xm(int var) { return (var >> 8) & 0xf; }
+MUFFT_HD constexpr int rv_twm(int var) { return (var >> 12) & 0xf; }
+MUFFT_HD constexpr int rv_T(int N, int var) { return N / rv_E(N, var); }
+MUFFT_HD constexpr int rv_B(int N, int var) { return rv_T(N, var) >= 256 ? 1 : 256 / rv_T(N, var); }
+MUFFT_HD constexpr int rv_WG(int N, int var) { return rv_T(N, var) * rv_B(N, var); }
+MUFFT_HD constexpr int rv_NP(int N, int var) {
+    return rv_E(N, var) >= N ? 1 : (ilog2(N) + ilog2(rv_E(N, var)) - 1) / ilog2(rv_E(N, var));
+}
+// LDS slots per transform (points for complex modes, floats for the split mode).
+MUFFT_HD constexpr int rv_lds_per(int N, int var) {
+    return rv_xm(var) == 2 ? N : N + N / rv_E(N, var);
+}
+MUFFT_HD constexpr long long rv_lds_bytes(int N, int var) {
+    return (long long)rv_B(N, var) * rv_lds_per(N, var) * (rv_xm(var) == 1 ? 4 : 8);
+}
+// Static choice per size, from a sweep of all registered variants on MI355X
+// (c2c, batches of 2^27 points): split-product twiddles pay off from 1024
+// points on; 16384 prefers 32 points per thread; 32768 must split the LDS
+// exchange to fit one workgroup.
+MUFFT_HD constexpr int rv_default(int N) {
+    return N <= 512 ? rv_make(16, 0, 0)
+         : N == 1024 ? rv_make(32, 0, 1)
+         : N <= 8192 ? rv_make(16, 0, 1)
+         : N == 16384 ? rv_make(32, 0, 1)
+         : rv_make(32, 1, 1);
+}
+
 // Radix of pass s for a radix sequence that uses E for all but the last pass.
 MUFFT_HD constexpr int seq_radix(int N, int E, int NP, int s) {
     return s < NP - 1 ? E : (N >> (ilog2(E) * (NP - 1)));
@@ -60,11 +96,10 @@ MUFFT_HD constexpr int seq_p(int E, int s) { return 1 << (ilog2(E) * s); }
 MUFFT_HD constexpr int col_E(int L) { return row_E(L); }
 MUFFT_HD constexpr int col_T(int L) { return L / col_E(L); }
 MUFFT_HD constexpr int col_TX(int L) {
-    return col_T(L) <= 32 ? 256 / col_T(L) : (col_T(L) <= 128 ? 8 : 4);
+    return col_T(L) <= 32 ? 256 / col_T(L) : (col_T(L) <= 128 ? 8 : (L >= 8192 ? 2 : 4));
 }
 MUFFT_HD constexpr int col_WG(int L) { return col_T(L) * col_TX(L); }
 MUFFT_HD constexpr int col_NP(int L) { return row_NP(L); }
-MUFFT_HD constexpr bool col_split(int L) { return L >= 8192; }
 MUFFT_HD constexpr int col_lds_rows(int L) { return L + L / col_E(L); }
 constexpr int COL_MAX_LOG2 = 13;   // fused column kernel covers L <= 8192
 constexpr int ROW_MAX_LOG2 = 15;   // fused row kernel covers N <= 32768

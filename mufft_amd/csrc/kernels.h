@@ -86,7 +86,11 @@ struct TwiddleArgs {
     long long n;
 };
 
-hipError_t launch_rows(int logN, int direction, int kind, const RowArgs &a, hipStream_t s);
+// Fused row kernel of size 2^logN in variant `var` (geometry.h, row_variants.h).
+hipError_t launch_rows(int logN, int direction, int kind, int var, const RowArgs &a, hipStream_t s);
+// Registered variants of a size; returns the count (at most max are written).
+int row_variant_list(int logN, int *vars, int max);
+bool row_variant_registered(int logN, int var);
 hipError_t launch_cols(int logL, int direction, const ColArgs &a, hipStream_t s);
 hipError_t launch_pass(int radix, int direction, const PassArgs &a, hipStream_t s);
 hipError_t launch_resolve_r2c(const ResolveArgs &a, hipStream_t s);
@@ -94,8 +98,5 @@ hipError_t launch_resolve_c2r(const ResolveArgs &a, hipStream_t s);
 hipError_t launch_mul(const MulArgs &a, hipStream_t s);
 hipError_t launch_twiddles(const TwiddleArgs &a, hipStream_t s);
 
-// Geometry queries for the planner.
-inline int rows_threads(int N) { return row_WG(N); }
-inline int rows_per_block(int N) { return row_B(N); }
 
 }  // namespace mufft

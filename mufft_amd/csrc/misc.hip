@@ -2,23 +2,65 @@
 // size, any strides), standalone resolve/multiply kernels and the FP64
 // twiddle generator.
 #include "kernels.h"
+#include "row_variants.h"
 #include "stockham.h"
 
 namespace mufft {
 
-const void *row_kernel_fwd(int logN, int kind);
-const void *row_kernel_inv(int logN, int kind);
+void row_kernels_c2c_fwd(const void **out);
+void row_kernels_c2c_inv(const void **out);
+void row_kernels_r2c(const void **out);
+void row_kernels_c2r(const void **out);
 
-hipError_t launch_rows(int logN, int direction, int kind, const RowArgs &a, hipStream_t s) {
-    const void *k = direction < 0 ? row_kernel_fwd(logN, kind) : row_kernel_inv(logN, kind);
-    if (!k) return hipErrorInvalidValue;
+namespace {
+struct RowTables {
+    const void *t[4][kNumRowVariants];
+    RowTables() {
+        row_kernels_c2c_fwd(t[0]);
+        row_kernels_c2c_inv(t[1]);
+        row_kernels_r2c(t[2]);
+        row_kernels_c2r(t[3]);
+    }
+};
+const RowTables &row_tables() {
+    static const RowTables tabs;
+    return tabs;
+}
+int row_variant_index(int logN, int var) {
+    for (int i = 0; i < kNumRowVariants; ++i)
+        if (kRowVariants[i].logN == logN && kRowVariants[i].var == var) return i;
+    return -1;
+}
+}  // namespace
+
+int row_variant_list(int logN, int *vars, int max) {
+    int n = 0;
+    for (int i = 0; i < kNumRowVariants; ++i) {
+        if (kRowVariants[i].logN != logN) continue;
+        if (vars && n < max) vars[n] = kRowVariants[i].var;
+        ++n;
+    }
+    return n;
+}
+
+bool row_variant_registered(int logN, int var) { return row_variant_index(logN, var) >= 0; }
+
+hipError_t launch_rows(int logN, int direction, int kind, int var, const RowArgs &a, hipStream_t s) {
+    const int idx = row_variant_index(logN, var);
+    if (idx < 0) return hipErrorInvalidValue;
+    int table;
+    if (kind == KIND_C2C) table = direction < 0 ? 0 : 1;
+    else if (kind == KIND_R2C && direction < 0) table = 2;
+    else if (kind == KIND_C2R && direction > 0) table = 3;
+    else return hipErrorInvalidValue;
+    const void *k = row_tables().t[table][idx];
     if (a.batch <= 0) return hipSuccess;
     const int N = 1 << logN;
-    const long long nblocks = ((long long)a.batch + row_B(N) - 1) / row_B(N);
+    const long long nblocks = ((long long)a.batch + rv_B(N, var) - 1) / rv_B(N, var);
     if (nblocks > 0x7fffffffLL) return hipErrorInvalidValue;
     RowArgs args = a;
     void *params[] = {&args};
-    return hipLaunchKernel(k, dim3((unsigned)nblocks), dim3(row_WG(N)), params, 0, s);
+    return hipLaunchKernel(k, dim3((unsigned)nblocks), dim3(rv_WG(N, var)), params, 0, s);
 }
 
 // ---------------------------------------------------------------------------

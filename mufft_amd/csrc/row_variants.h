@@ -1,0 +1,47 @@
+// Registered variants of the fused row kernel (host and device).
+//
+// Each entry (log2 N, variant) is instantiated for c2c forward/inverse, r2c
+// and c2r.  The planner's static choice is rv_default(N) (geometry.h); the
+// autotuner times the other entries of a size on the device and may pick one.
+#pragma once
+
+#include "geometry.h"
+
+namespace mufft {
+
+struct RowVariantEntry {
+    int logN;
+    int var;
+};
+
+// Shorthands: E points per thread, exchange (P padded, S split, F flat),
+// twiddles (full table or split product).
+constexpr int V16P = rv_make(16, 0, 0);
+constexpr int V16PT = rv_make(16, 0, 1);
+constexpr int V16S = rv_make(16, 1, 0);
+constexpr int V16F = rv_make(16, 2, 0);
+constexpr int V8P = rv_make(8, 0, 0);
+constexpr int V32P = rv_make(32, 0, 0);
+constexpr int V32PT = rv_make(32, 0, 1);
+constexpr int V32S = rv_make(32, 1, 0);
+constexpr int V32ST = rv_make(32, 1, 1);
+constexpr int V64ST = rv_make(64, 1, 1);
+constexpr int V64PT = rv_make(64, 0, 1);
+
+constexpr RowVariantEntry kRowVariants[] = {
+    {1, V16P}, {2, V16P}, {3, V16P}, {4, V16P},
+    {5, V16P}, {5, V8P},
+    {6, V16P}, {6, V8P},
+    {7, V16P}, {7, V8P},
+    {8, V16P}, {8, V8P},
+    {9, V16P}, {9, V16PT}, {9, V8P}, {9, V32PT},
+    {10, V16P}, {10, V16PT}, {10, V8P}, {10, V16S}, {10, V16F}, {10, V32PT},
+    {11, V16P}, {11, V16PT}, {11, V8P}, {11, V16S}, {11, V16F}, {11, V32PT},
+    {12, V16P}, {12, V16PT}, {12, V8P}, {12, V16S}, {12, V16F}, {12, V32PT},
+    {13, V32P}, {13, V32PT}, {13, V16PT}, {13, V16S},
+    {14, V32P}, {14, V32PT}, {14, V16PT}, {14, V64PT},
+    {15, V32S}, {15, V32ST}, {15, V64ST},
+};
+constexpr int kNumRowVariants = sizeof(kRowVariants) / sizeof(kRowVariants[0]);
+
+}  // namespace mufft

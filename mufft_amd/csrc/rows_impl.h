@@ -10,19 +10,26 @@
 #pragma once
 
 #include "kernels.h"
+#include "row_variants.h"
 #include "stockham.h"
 
 namespace mufft {
 
-template <int N, int DIR, int KIND>
-__global__ void __launch_bounds__(row_WG(N)) fft_rows_kernel(RowArgs a) {
-    constexpr int E = row_E(N);
+template <int N, int DIR, int KIND, int VAR>
+__global__ void __launch_bounds__(rv_WG(N, VAR)) fft_rows_kernel(RowArgs a) {
+    constexpr int E = rv_E(N, VAR);
     constexpr int LOGE = ilog2(E);
-    constexpr int T = row_T(N);
-    constexpr int B = row_B(N);
-    constexpr bool SPLIT = row_split(N);
-    constexpr int LDSN = row_lds_per(N);
-    constexpr int SMEM = SPLIT ? (LDSN + 1) / 2 : B * LDSN;
+    constexpr int T = rv_T(N, VAR);
+    constexpr int B = rv_B(N, VAR);
+    constexpr int XM = rv_xm(VAR);
+    constexpr int TWM = rv_twm(VAR);
+    constexpr bool SPLIT = XM == XM_SPLIT;
+    constexpr int LDSN = rv_lds_per(N, VAR);
+    constexpr int SMEM = SPLIT ? (B * LDSN + 1) / 2 : B * LDSN;
+    // With T >= 64 every wave works on one transform, so its base addresses
+    // are wave-uniform and the strided traffic can go through buffer
+    // descriptors (one VGPR offset per thread, scalar offsets per slot).
+    constexpr bool BUF = T >= 64;
     __shared__ float2 smem[SMEM];
 
     const int tid = threadIdx.x;
@@ -30,30 +37,60 @@ __global__ void __launch_bounds__(row_WG(N)) fft_rows_kernel(RowArgs a) {
     const int sub = tid / T;
     const long long g = (long long)blockIdx.x * B + sub;
     const bool active = g < a.batch;
-    float2 *L = smem + (SPLIT ? 0 : sub * LDSN);
-    float *Lf = reinterpret_cast<float *>(smem);
-    auto slot = [](int o) { return pad_slot<LOGE>(o); };
+    float2 *L = smem + sub * LDSN;
+    float *Lf = reinterpret_cast<float *>(smem) + sub * LDSN;
+    auto slot = [](int o) { return XM == XM_FLAT ? o : pad_slot<LOGE>(o); };
 
-    float2 v[E];
     const unsigned flags = a.flags;
     const bool zp = (flags & RF_ZERO_PAD) != 0;
+    const bool mul = (flags & RF_MUL) != 0;
+    const bool cj = (flags & RF_CONJ2) != 0;
+    const bool full = (flags & RF_FULL) != 0;
 
+    const float2 *X = a.in + g * a.idist;
+    const float2 *Y = a.in2 + g * a.idist2;
+    float2 *Xo = a.out + g * a.odist;
+    // Element vp + sp of an operand: vp per lane, sp a compile-time slot offset.
+    __amdgpu_buffer_rsrc_t rin, rin2, rout, rrtw;  // set and used only when BUF
+    if constexpr (BUF) {
+        const unsigned nin = KIND == KIND_C2R ? N + 1 : ((zp && !mul) ? N / 2 : N);
+        const unsigned nout = KIND == KIND_R2C ? (full ? 2 * N : N + 1) : N;
+        rin = make_rsrc(X, active ? nin * 8u : 0u);
+        rin2 = make_rsrc(Y, (active && mul) ? (KIND == KIND_C2R ? N + 1 : N) * 8u : 0u);
+        rout = make_rsrc(Xo, active ? nout * 8u : 0u);
+        rrtw = make_rsrc(a.rtw, KIND == KIND_C2C ? 0u : N * 8u);
+    }
+    auto ld_in = [&](int vp, int sp) -> float2 {
+        if constexpr (BUF) return bload(rin, (unsigned)vp * 8u, (unsigned)sp * 8u);
+        else return X[vp + sp];
+    };
+    auto ld_in2 = [&](int vp, int sp) -> float2 {
+        if constexpr (BUF) return bload(rin2, (unsigned)vp * 8u, (unsigned)sp * 8u);
+        else return Y[vp + sp];
+    };
+    auto ld_rtw = [&](int vp, int sp) -> float2 {
+        if constexpr (BUF) return bload(rrtw, (unsigned)vp * 8u, (unsigned)sp * 8u);
+        else return a.rtw[vp + sp];
+    };
+    auto st_out = [&](int vp, int sp, float2 val) {
+        if constexpr (BUF) bstore(rout, (unsigned)vp * 8u, (unsigned)sp * 8u, val);
+        else Xo[vp + sp] = val;
+    };
+
+    float2 v[E];
     if constexpr (KIND == KIND_C2R) {
         // Pre-pass resolve (kernel.c:36-45), optionally after the fused multiply.
-        const float2 *X = a.in + g * a.idist;
-        const float2 *Y = a.in2 + g * a.idist2;
-        const bool mul = (flags & RF_MUL) != 0;
-        const bool cj = (flags & RF_CONJ2) != 0;
+        // X[N - k] for k = lt + T*c is element (T - lt) + T*(E-1-c).
         static_for<0, E>([&](auto C) {
             constexpr int c = decltype(C)::value;
             const int k = lt + T * c;
             float2 z = make_float2(0.f, 0.f);
-            if (active && !(zp && c >= E / 2)) {
-                float2 xa = X[k];
-                float2 xb = X[N - k];
+            if ((BUF || active) && !(zp && c >= E / 2)) {
+                float2 xa = ld_in(lt, T * c);
+                float2 xb = ld_in(T - lt, T * (E - 1 - c));
                 if (mul) {
-                    float2 ya = Y[k];
-                    float2 yb = Y[N - k];
+                    float2 ya = ld_in2(lt, T * c);
+                    float2 yb = ld_in2(T - lt, T * (E - 1 - c));
                     if (cj) {
                         ya = cconj(ya);
                         yb = cconj(yb);
@@ -62,50 +99,51 @@ __global__ void __launch_bounds__(row_WG(N)) fft_rows_kernel(RowArgs a) {
                     xb = cscale(cmul(xb, yb), a.norm);
                 }
                 const float2 bc = cconj(xb);
-                z = cadd(cadd(xa, bc), cmul(csub(xa, bc), a.rtw[k]));
+                z = cadd(cadd(xa, bc), cmul(csub(xa, bc), ld_rtw(lt, T * c)));
             }
+            (void)k;
             v[c] = z;
         });
     } else {
-        const float2 *X = a.in + g * a.idist;
-        if (KIND == KIND_C2C && (flags & RF_MUL)) {
-            const float2 *Y = a.in2 + g * a.idist2;
-            const bool cj = (flags & RF_CONJ2) != 0;
+        if (KIND == KIND_C2C && mul) {
             static_for<0, E>([&](auto C) {
                 constexpr int c = decltype(C)::value;
                 float2 z = make_float2(0.f, 0.f);
-                if (active) {
-                    float2 y = Y[lt + T * c];
+                if (BUF || active) {
+                    float2 y = ld_in2(lt, T * c);
                     if (cj) y = cconj(y);
-                    z = cscale(cmul(X[lt + T * c], y), a.norm);
+                    z = cscale(cmul(ld_in(lt, T * c), y), a.norm);
                 }
                 v[c] = z;
             });
         } else {
             static_for<0, E>([&](auto C) {
                 constexpr int c = decltype(C)::value;
-                v[c] = (active && !(zp && c >= E / 2)) ? X[lt + T * c] : make_float2(0.f, 0.f);
+                v[c] = ((BUF || active) && !(zp && c >= E / 2)) ? ld_in(lt, T * c) : make_float2(0.f, 0.f);
             });
         }
     }
 
-    stockham_passes<N, DIR, SPLIT>(v, lt, a.tw, L, Lf, slot);
+    if constexpr (BUF) {
+        const unsigned ntw = rv_NP(N, VAR) > 1 ? (unsigned)(N - E) : 0u;
+        stockham_passes<N, E, DIR, XM, TWM>(v, lt, TwBuffer{make_rsrc(a.tw, ntw * 8u)}, L, Lf, slot);
+    } else {
+        stockham_passes<N, E, DIR, XM, TWM>(v, lt, TwGlobal{a.tw}, L, Lf, slot);
+    }
 
-    float2 *Xo = a.out + g * a.odist;
     if constexpr (KIND == KIND_R2C) {
         // Post-pass resolve (kernel.c:47-82): bin k needs Z[k] and Z[N - k],
         // which another thread holds, so Z goes through LDS once more.
-        const bool full = (flags & RF_FULL) != 0;
-        auto emit = [&](int c, int k, float2 za, float2 zp_) {
-            if (k == 0) {
-                Xo[0] = make_float2(za.x + za.y, 0.f);
-                Xo[N] = make_float2(za.x - za.y, 0.f);
+        auto emit = [&](int c_sp, float2 za, float2 zn, bool dc) {
+            if (dc) {
+                st_out(0, 0, make_float2(za.x + za.y, 0.f));
+                st_out(0, N, make_float2(za.x - za.y, 0.f));
             } else {
-                const float2 zb = cconj(zp_);
+                const float2 zb = cconj(zn);
                 const float2 fe = cadd(za, zb);
-                const float2 fo = cmul(a.rtw[k], csub(za, zb));
-                Xo[k] = cscale(cadd(fe, fo), 0.5f);
-                if (full) Xo[k + N] = cscale(csub(fe, fo), 0.5f);
+                const float2 fo = cmul(ld_rtw(lt, c_sp), csub(za, zb));
+                st_out(lt, c_sp, cscale(cadd(fe, fo), 0.5f));
+                if (full) st_out(lt, c_sp + N, cscale(csub(fe, fo), 0.5f));
             }
         };
         if constexpr (!SPLIT) {
@@ -114,67 +152,69 @@ __global__ void __launch_bounds__(row_WG(N)) fft_rows_kernel(RowArgs a) {
                 L[slot(lt + T * c)] = v[c];
             });
             __syncthreads();
-            if (active) {
+            if (BUF || active) {
                 static_for<0, E>([&](auto C) {
                     constexpr int c = decltype(C)::value;
                     const int k = lt + T * c;
-                    emit(c, k, v[c], L[slot((N - k) & (N - 1))]);
+                    emit(T * c, v[c], L[slot((N - k) & (N - 1))], c == 0 && lt == 0);
                 });
             }
         } else {
-            float px[E];
-            static_for<0, E>([&](auto C) {
+            // The LDS of the split exchange holds half the spectrum as complex
+            // points.  Round A publishes the upper half (slots c >= E/2) and
+            // resolves the lower half; round B the other way round.  Partners
+            // are consumed as they are read, so no partner array stays live.
+            // Z[N/2] is its own partner (lt == 0, c == E/2).
+            float2 *Lc = reinterpret_cast<float2 *>(Lf);
+            static_for<E / 2, E>([&](auto C) {
                 constexpr int c = decltype(C)::value;
-                Lf[slot(lt + T * c)] = v[c].x;
+                Lc[slot(lt + T * c - N / 2)] = v[c];
             });
             __syncthreads();
-            static_for<0, E>([&](auto C) {
-                constexpr int c = decltype(C)::value;
-                px[c] = Lf[slot((N - (lt + T * c)) & (N - 1))];
-            });
-            __syncthreads();
-            static_for<0, E>([&](auto C) {
-                constexpr int c = decltype(C)::value;
-                Lf[slot(lt + T * c)] = v[c].y;
-            });
-            __syncthreads();
-            if (active) {
-                static_for<0, E>([&](auto C) {
+            if (BUF || active) {
+                static_for<0, E / 2>([&](auto C) {
                     constexpr int c = decltype(C)::value;
                     const int k = lt + T * c;
-                    emit(c, k, v[c], make_float2(px[c], Lf[slot((N - k) & (N - 1))]));
+                    const bool dc = c == 0 && lt == 0;
+                    // Select values, not references: a select between a pointer
+                    // into v[] and an LDS pointer would force v[] into scratch.
+                    const float2 zn = Lc[slot(dc ? 0 : N - k - N / 2)];
+                    emit(T * c, v[c], dc ? v[c] : zn, dc);
+                });
+            }
+            __syncthreads();
+            static_for<0, E / 2>([&](auto C) {
+                constexpr int c = decltype(C)::value;
+                Lc[slot(lt + T * c)] = v[c];
+            });
+            __syncthreads();
+            if (BUF || active) {
+                static_for<E / 2, E>([&](auto C) {
+                    constexpr int c = decltype(C)::value;
+                    const int k = lt + T * c;
+                    const bool self = c == E / 2 && lt == 0;
+                    const float2 zn = Lc[slot(self ? 0 : N - k)];
+                    emit(T * c, v[c], self ? v[c] : zn, false);
                 });
             }
         }
     } else {
-        if (active) {
+        if (BUF || active) {
             static_for<0, E>([&](auto C) {
                 constexpr int c = decltype(C)::value;
-                Xo[lt + T * c] = v[c];
+                st_out(lt, T * c, v[c]);
             });
         }
     }
 }
 
-template <int DIR, int KIND>
-static const void *row_kernel_ptr(int logN) {
-    switch (logN) {
-    case 1: return (const void *)&fft_rows_kernel<2, DIR, KIND>;
-    case 2: return (const void *)&fft_rows_kernel<4, DIR, KIND>;
-    case 3: return (const void *)&fft_rows_kernel<8, DIR, KIND>;
-    case 4: return (const void *)&fft_rows_kernel<16, DIR, KIND>;
-    case 5: return (const void *)&fft_rows_kernel<32, DIR, KIND>;
-    case 6: return (const void *)&fft_rows_kernel<64, DIR, KIND>;
-    case 7: return (const void *)&fft_rows_kernel<128, DIR, KIND>;
-    case 8: return (const void *)&fft_rows_kernel<256, DIR, KIND>;
-    case 9: return (const void *)&fft_rows_kernel<512, DIR, KIND>;
-    case 10: return (const void *)&fft_rows_kernel<1024, DIR, KIND>;
-    case 11: return (const void *)&fft_rows_kernel<2048, DIR, KIND>;
-    case 12: return (const void *)&fft_rows_kernel<4096, DIR, KIND>;
-    case 13: return (const void *)&fft_rows_kernel<8192, DIR, KIND>;
-    case 14: return (const void *)&fft_rows_kernel<16384, DIR, KIND>;
-    case 15: return (const void *)&fft_rows_kernel<32768, DIR, KIND>;
-    default: return nullptr;
+// Table of the instantiated variants of one (direction, kind).
+template <int DIR, int KIND, int I = 0>
+void fill_row_kernels(const void **out) {
+    if constexpr (I < kNumRowVariants) {
+        constexpr RowVariantEntry e = kRowVariants[I];
+        out[I] = reinterpret_cast<const void *>(&fft_rows_kernel<(1 << e.logN), DIR, KIND, e.var>);
+        fill_row_kernels<DIR, KIND, I + 1>(out);
     }
 }
 

@@ -7,7 +7,7 @@
 // takes an r-point DFT and writes output q to (b - k)*r + k + q*p.  Unlike the
 // reference, which runs one radix-2/4/8 pass per sweep over memory, all passes
 // of a transform run inside one workgroup: each thread keeps E points in
-// registers, butterflies are in-register DFTs of radix up to 32, and points
+// registers, butterflies are in-register DFTs of radix up to 64, and points
 // move between passes through LDS only.
 #pragma once
 
@@ -15,17 +15,78 @@
 
 namespace mufft {
 
+// Exchange modes between passes.
+enum XMode : int {
+    XM_PAD = 0,    // complex points, one spare LDS slot every E points
+    XM_SPLIT = 1,  // real parts then imaginary parts through a float array (half the LDS)
+    XM_FLAT = 2,   // complex points, no padding (smallest footprint)
+};
+
+// Twiddle application: TW_FULL loads every exp(dir*2*pi*i*k*m/(r*p)) from the
+// table; TW_SPLIT loads only m < M and multiples of M (M = 8 for r = 64/32,
+// 4 for r = 16) and forms the rest as one product, cutting the loads per
+// radix-32 butterfly from 31 to 10 at the cost of about one extra rounding.
+enum TwMode : int { TW_FULL = 0, TW_SPLIT = 1 };
+
+template <int r>
+constexpr int tw_split_m() { return r >= 32 ? 8 : (r >= 16 ? 4 : r); }
+
+// Multiplies u[m] by the pass twiddle exp(dir*2*pi*i*k*m/(r*p)) = table
+// entry base + (m-1)*p + k, read through twl(index, k) = table[index + k].
+template <int r, int p, int TWM, typename TwLoad>
+__device__ __forceinline__ void apply_twiddles(float2 (&u)[r], int k, int base, TwLoad twl) {
+    if constexpr (TWM == TW_SPLIT && r >= 16) {
+        constexpr int ML = tw_split_m<r>();
+        float2 lo[ML];
+        static_for<1, ML>([&](auto M) {
+            constexpr int m = decltype(M)::value;
+            lo[m] = twl(base + (m - 1) * p, k);
+            u[m] = cmul(u[m], lo[m]);
+        });
+        static_for<1, r / ML>([&](auto H) {
+            constexpr int mh = decltype(H)::value;
+            const float2 hi = twl(base + (mh * ML - 1) * p, k);
+            u[mh * ML] = cmul(u[mh * ML], hi);
+            static_for<1, ML>([&](auto M) {
+                constexpr int m = decltype(M)::value;
+                u[mh * ML + m] = cmul(u[mh * ML + m], cmul(lo[m], hi));
+            });
+        });
+    } else {
+        static_for<1, r>([&](auto M) {
+            constexpr int m = decltype(M)::value;
+            u[m] = cmul(u[m], twl(base + (m - 1) * p, k));
+        });
+    }
+}
+
+// Twiddle-table readers: plain global loads, or buffer loads with the table
+// index as a scalar offset.
+struct TwGlobal {
+    const float2 *__restrict__ t;
+    __device__ __forceinline__ float2 operator()(int idx, int k) const { return t[idx + k]; }
+};
+struct TwBuffer {
+    __amdgpu_buffer_rsrc_t r;
+    __device__ __forceinline__ float2 operator()(int idx, int k) const {
+        return bload(r, (unsigned)k * 8u, (unsigned)idx * 8u);
+    }
+};
+
+// Number of passes when every pass but the last has radix E.
+MUFFT_HD constexpr int np_for(int N, int E) {
+    return E >= N ? 1 : (ilog2(N) + ilog2(E) - 1) / ilog2(E);
+}
+
 // v[c] holds point lt + T*c on entry (T = N/E) and holds output point lt + T*c
-// on exit.  `tw` is the twiddle table of the sequence (E, ..., E, r_last).
+// on exit.  `twl` reads the twiddle table of the sequence (E, ..., E, r_last).
 // `Slot(o)` maps a point index to its LDS slot; `L` is the complex LDS view,
 // `Lf` the float view used by the split exchange.
-template <int N, int DIR, bool SPLIT, typename Slot>
-__device__ __forceinline__ void stockham_passes(float2 (&v)[row_E(N)], int lt,
-                                                const float2 *__restrict__ tw,
+template <int N, int E, int DIR, int XM, int TWM, typename TwLoad, typename Slot>
+__device__ __forceinline__ void stockham_passes(float2 (&v)[E], int lt, TwLoad twl,
                                                 float2 *L, float *Lf, Slot slot) {
-    constexpr int E = row_E(N);
     constexpr int T = N / E;
-    constexpr int NP = row_NP(N);
+    constexpr int NP = np_for(N, E);
     static_for<0, NP>([&](auto S) {
         constexpr int s = decltype(S)::value;
         constexpr int r = seq_radix(N, E, NP, s);
@@ -37,11 +98,7 @@ __device__ __forceinline__ void stockham_passes(float2 (&v)[row_E(N)], int lt,
             static_for<0, r>([&](auto M) { u[decltype(M)::value] = v[j + decltype(M)::value * J]; });
             if constexpr (s > 0) {
                 const int k = (lt + T * j) & (p - 1);
-                const float2 *t = tw + tw_offset(E, p) + k;
-                static_for<1, r>([&](auto M) {
-                    constexpr int m = decltype(M)::value;
-                    u[m] = cmul(u[m], t[(m - 1) * p]);
-                });
+                apply_twiddles<r, p, TWM>(u, k, tw_offset(E, p), twl);
             }
             dft<r, DIR>(u);
             static_for<0, r>([&](auto Q) { v[j + decltype(Q)::value * J] = u[decltype(Q)::value]; });
@@ -51,7 +108,7 @@ __device__ __forceinline__ void stockham_passes(float2 (&v)[row_E(N)], int lt,
             // (lt - k)*E + k + q*p; the next pass reads points lt + T*c.
             const int k = lt & (p - 1);
             const int base = (lt - k) * E + k;
-            if constexpr (!SPLIT) {
+            if constexpr (XM != XM_SPLIT) {
                 static_for<0, E>([&](auto Q) {
                     constexpr int q = decltype(Q)::value;
                     L[slot(base + q * p)] = v[q];

@@ -14,10 +14,8 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def host_table(N, d):
-    E = 16 if N <= 4096 else 32
-    if N <= 16:
-        E = N
+def host_table(N, d, E):
+    E = min(E, N)
     radix = []
     left = N
     while left > 1:
@@ -40,8 +38,10 @@ def host_table(N, d):
 @pytest.mark.parametrize("d", [-1, 1])
 def test_twiddle_table_bit_exact(gpu, N, d):
     import mufft_amd as M
+    from mufft_amd import _lib
+    E = _lib.load().mufft_hip_default_variant(N, 0) & 0xFF  # points per thread of the default kernel
     t = M.twiddle_table(N, d)
-    h = host_table(N, d)
+    h = host_table(N, d, E)
     assert t.shape == h.shape
     mism = int(np.count_nonzero(t.view(np.uint32) != h.view(np.uint32)))
     assert mism == 0, f"{mism} FP32 mismatches of {2 * h.size}"

@@ -1,0 +1,104 @@
+"""Every registered fused-kernel variant is correct, and the autotuner works.
+
+The autotuner may select any registered variant (points per thread, LDS
+exchange mode, twiddle scheme), so each one is checked here against FP64 for
+every transform kind, batch > 1 and a partial last workgroup.
+"""
+import numpy as np
+import pytest
+
+from oracle import fp64
+from tests.helpers import abs_tol, cplx, empty_dev, from_dev, real, rel_rms, to_dev
+
+pytestmark = pytest.mark.gpu
+
+KINDS = {0: ("c2c", -1), 1: ("c2c", 1), 2: ("r2c", -1), 3: ("c2r", 1)}
+
+
+def _variants(N, kind):
+    import ctypes
+    from mufft_amd import _lib
+    L = _lib.load()
+    buf = (ctypes.c_int * 64)()
+    n = L.mufft_hip_list_variants(N, kind, buf, 64)
+    assert n > 0
+    return list(buf[:n])
+
+
+def _run(gpu, kind, N, x, batch):
+    import mufft_amd as M
+    import torch
+    name, d = KINDS[kind]
+    if name == "c2c":
+        p = M.Plan1D("c2c", N, d)
+        dy = empty_dev((batch, N), np.complex64, gpu)
+    elif name == "r2c":
+        p = M.Plan1D("r2c", N)
+        dy = empty_dev((batch, N // 2 + 1), np.complex64, gpu)
+    else:
+        p = M.Plan1D("c2r", N)
+        dy = empty_dev((batch, N), np.float32, gpu)
+    p.execute_batched(dy, to_dev(x, gpu), batch)
+    torch.cuda.synchronize()
+    return from_dev(dy)
+
+
+@pytest.mark.parametrize("logn", range(1, 16))
+@pytest.mark.parametrize("kind", [0, 1, 2, 3])
+def test_every_variant(gpu, logn, kind):
+    from mufft_amd import _lib
+    L = _lib.load()
+    name, d = KINDS[kind]
+    N = 1 << logn if name == "c2c" else 1 << (logn + 1)   # real sizes for r2c / c2r
+    batch = 3
+    if name == "c2c":
+        x = cplx(logn * 11 + kind, batch * N, 0.5).reshape(batch, N)
+        ref = fp64.c2c(x, d)
+    elif name == "r2c":
+        x = real(logn * 13 + kind, batch * N, 0.5).reshape(batch, N)
+        ref = fp64.r2c(x)
+    else:
+        x = cplx(logn * 17 + kind, batch * (N // 2 + 1), 0.5).reshape(batch, N // 2 + 1)
+        ref = fp64.c2r(x, N)
+    try:
+        for var in _variants(N, kind):
+            assert L.mufft_hip_set_variant(N, kind, var) == 0
+            assert L.mufft_hip_get_variant(N, kind) == var
+            y = _run(gpu, kind, N, x, batch)
+            assert rel_rms(y, ref) < 2e-6, (N, kind, hex(var))
+            assert np.max(np.abs(y - ref)) < abs_tol(N, factor=2e-6), (N, kind, hex(var))
+    finally:
+        L.mufft_hip_set_variant(N, kind, 0)
+
+
+@pytest.mark.parametrize("N,kind", [(4096, 0), (1024, 1), (65536, 2), (65536, 3)])
+def test_autotune_selects_valid_variant(gpu, tmp_path, N, kind):
+    import ctypes
+    from mufft_amd import _lib
+    L = _lib.load()
+    us = ctypes.c_double(0)
+    try:
+        best = L.mufft_hip_autotune_1d(N, kind, 64, ctypes.byref(us))
+        assert best > 0, _lib.last_error()
+        assert best in _variants(N, kind)
+        assert L.mufft_hip_get_variant(N, kind) == best and us.value > 0
+        f = tmp_path / "tune.txt"
+        assert L.mufft_hip_tune_save(str(f).encode()) == 0
+        assert L.mufft_hip_set_variant(N, kind, 0) == 0
+        assert L.mufft_hip_tune_load(str(f).encode()) >= 1
+        assert L.mufft_hip_get_variant(N, kind) == best
+        # a plan created now runs the tuned variant and stays correct
+        name, d = KINDS[kind]
+        batch = 2
+        if name == "c2c":
+            x = cplx(5, batch * N, 0.5).reshape(batch, N)
+            ref = fp64.c2c(x, d)
+        elif name == "r2c":
+            x = real(5, batch * N, 0.5).reshape(batch, N)
+            ref = fp64.r2c(x)
+        else:
+            x = cplx(5, batch * (N // 2 + 1), 0.5).reshape(batch, N // 2 + 1)
+            ref = fp64.c2r(x, N)
+        assert rel_rms(_run(gpu, kind, N, x, batch), ref) < 2e-6
+    finally:
+        L.mufft_hip_set_variant(N, kind, 0)
