@@ -83,6 +83,38 @@ int mufft_hip_autotune_1d(unsigned N, int kind, unsigned howmany, double *best_u
 int mufft_hip_tune_save(const char *path);
 int mufft_hip_tune_load(const char *path);
 
+/* ---- Uneven convolution batch ----
+ * Many fast convolutions of mixed sizes and methods in one persistent kernel
+ * launch fed by a device work queue, replayed from a captured hipGraph.  Each
+ * job is one mufft_plan_conv round (both inputs transformed, multiplied,
+ * inverse transformed, normalised by 1/N) on device buffers:
+ *   first:  N/2 reals (mono) or N/2 complex (stereo) with the matching
+ *           MUFFT_CONV_METHOD_FLAG_ZERO_PAD_UPPER_HALF_FIRST bit, else N;
+ *   second: N/2 reals with ..._SECOND, else N reals;
+ *   output: N reals (mono) or N complex (stereo).
+ * Mono N = 8..65536, stereo N = 8..32768, powers of two.  The batch owns the
+ * spectra scratch; the job pointers are fixed at creation. */
+typedef struct mufft_hip_conv_job {
+    unsigned N;
+    unsigned method;
+    const void *first;
+    const void *second;
+    void *output;
+} mufft_hip_conv_job;
+
+typedef struct mufft_hip_conv_batch mufft_hip_conv_batch;
+
+mufft_hip_conv_batch *mufft_hip_create_conv_batch(const mufft_hip_conv_job *jobs, unsigned count,
+                                                  unsigned flags);
+/* Replays the captured graph on stream (NULL: the batch's stream, and waits). */
+int mufft_hip_execute_conv_batch(mufft_hip_conv_batch *batch, void *stream);
+/* Same kernels without the graph. */
+int mufft_hip_execute_conv_batch_direct(mufft_hip_conv_batch *batch, void *stream);
+/* 0 = last execution complete, 1 = a dependency wait timed out. */
+int mufft_hip_conv_batch_status(mufft_hip_conv_batch *batch);
+int mufft_hip_conv_batch_info(const mufft_hip_conv_batch *batch, int *ntasks, int *grid);
+void mufft_hip_free_conv_batch(mufft_hip_conv_batch *batch);
+
 #ifdef __cplusplus
 }
 #endif

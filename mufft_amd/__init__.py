@@ -10,6 +10,7 @@ _lib.load()
 
 from .api import (  # noqa: E402
     CONV_BLOCK_FIRST,
+    ConvBatch,
     CONV_BLOCK_SECOND,
     CONV_METHOD_FLAG_MONO_MONO,
     CONV_METHOD_FLAG_STEREO_MONO,
@@ -35,5 +36,5 @@ __all__ = [
     "FLAG_ZERO_PAD_UPPER_HALF", "CONV_METHOD_FLAG_MONO_MONO", "CONV_METHOD_FLAG_STEREO_MONO",
     "CONV_METHOD_FLAG_ZERO_PAD_UPPER_HALF_FIRST", "CONV_METHOD_FLAG_ZERO_PAD_UPPER_HALF_SECOND",
     "CONV_BLOCK_FIRST", "CONV_BLOCK_SECOND", "MufftError", "Plan1D", "Plan2D", "PlanConv",
-    "device_count", "get_convolve_func", "twiddle_table",
+    "device_count", "get_convolve_func", "twiddle_table", "ConvBatch",
 ]

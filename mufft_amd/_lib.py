@@ -57,7 +57,18 @@ SIGNATURES = [
     ("mufft_hip_autotune_1d", _i, [_u, _i, _u, ctypes.POINTER(ctypes.c_double)]),
     ("mufft_hip_tune_save", _i, [ctypes.c_char_p]),
     ("mufft_hip_tune_load", _i, [ctypes.c_char_p]),
+    ("mufft_hip_create_conv_batch", _vp, [_vp, _u, _u]),
+    ("mufft_hip_execute_conv_batch", _i, [_vp, _vp]),
+    ("mufft_hip_execute_conv_batch_direct", _i, [_vp, _vp]),
+    ("mufft_hip_conv_batch_status", _i, [_vp]),
+    ("mufft_hip_conv_batch_info", _i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+    ("mufft_hip_free_conv_batch", None, [_vp]),
 ]
+
+
+class ConvJob(ctypes.Structure):
+    """mufft_hip_conv_job (include/mufft/mufft_hip.h)."""
+    _fields_ = [("N", _u), ("method", _u), ("first", _vp), ("second", _vp), ("output", _vp)]
 
 _lib = None
 

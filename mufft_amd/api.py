@@ -224,6 +224,48 @@ def twiddle_table(N: int, direction: int) -> np.ndarray:
     return out[:size]
 
 
+class ConvBatch:
+    """mufft_hip_conv_batch: uneven convolution jobs in one persistent launch.
+
+    ``jobs`` is a list of (N, method, first, second, output) with device
+    buffers (torch tensors or addresses) laid out as in mufft_plan_conv."""
+
+    def __init__(self, jobs, flags: int = 0):
+        L = _lib.load()
+        arr = (_lib.ConvJob * len(jobs))()
+        for i, (N, method, first, second, out) in enumerate(jobs):
+            arr[i] = _lib.ConvJob(N, method, ptr(first), ptr(second), ptr(out))
+        self._keep = jobs
+        h = L.mufft_hip_create_conv_batch(arr, len(jobs), flags)
+        if not h:
+            raise MufftError(f"mufft_hip_create_conv_batch failed: {_lib.last_error()}")
+        self._h = h
+
+    def execute(self, stream=None, graph: bool = True):
+        L = _lib.load()
+        fn = L.mufft_hip_execute_conv_batch if graph else L.mufft_hip_execute_conv_batch_direct
+        _check(fn(self._h, stream_handle(stream)), "mufft_hip_execute_conv_batch")
+
+    def status(self) -> int:
+        return int(_lib.load().mufft_hip_conv_batch_status(self._h))
+
+    def info(self) -> tuple[int, int]:
+        nt, g = ctypes.c_int(0), ctypes.c_int(0)
+        _lib.load().mufft_hip_conv_batch_info(self._h, ctypes.byref(nt), ctypes.byref(g))
+        return nt.value, g.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.load().mufft_hip_free_conv_batch(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 KIND_CODES = {"c2c_forward": 0, "c2c_inverse": 1, "r2c": 2, "c2r": 3}
 
 

@@ -14,7 +14,7 @@ INCLUDE = ROOT / "include"
 BUILD = PKG / "_build"
 LIB = PKG / "libmufft_hip.so"
 
-SOURCES = ["rows_c2c_fwd.hip", "rows_c2c_inv.hip", "rows_r2c.hip", "rows_c2r.hip", "api.hip", "misc.hip", "cols.hip"]
+SOURCES = ["rows_c2c_fwd.hip", "rows_c2c_inv.hip", "rows_r2c.hip", "rows_c2r.hip", "conv_batch.hip", "api.hip", "misc.hip", "cols.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 # Packed-FP32 instructions are disabled for all device code.

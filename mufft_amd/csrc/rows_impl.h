@@ -15,41 +15,32 @@
 
 namespace mufft {
 
+// One fused transform (N points, variant VAR) by the T = N/E threads of a
+// workgroup slot.  X / Y / Xo are this transform's input, second operand and
+// output; L / Lf its LDS region (rv_lds_per(N, VAR) slots).  Every thread of
+// the workgroup must call it (the exchanges synchronise the workgroup);
+// inactive slots pass active = false.  With T >= 64 a wave never spans two
+// transforms, so the pointers are wave-uniform and the strided traffic goes
+// through buffer descriptors (one VGPR offset per thread, scalar offsets per
+// slot).
 template <int N, int DIR, int KIND, int VAR>
-__global__ void __launch_bounds__(rv_WG(N, VAR)) fft_rows_kernel(RowArgs a) {
+__device__ __forceinline__ void row_transform(const float2 *X, const float2 *Y, float2 *Xo,
+                                              bool active, int lt, const float2 *tw,
+                                              const float2 *rtw, unsigned flags, float norm,
+                                              float2 *L, float *Lf) {
     constexpr int E = rv_E(N, VAR);
     constexpr int LOGE = ilog2(E);
     constexpr int T = rv_T(N, VAR);
-    constexpr int B = rv_B(N, VAR);
     constexpr int XM = rv_xm(VAR);
     constexpr int TWM = rv_twm(VAR);
     constexpr bool SPLIT = XM == XM_SPLIT;
-    constexpr int LDSN = rv_lds_per(N, VAR);
-    constexpr int SMEM = SPLIT ? (B * LDSN + 1) / 2 : B * LDSN;
-    // With T >= 64 every wave works on one transform, so its base addresses
-    // are wave-uniform and the strided traffic can go through buffer
-    // descriptors (one VGPR offset per thread, scalar offsets per slot).
     constexpr bool BUF = T >= 64;
-    __shared__ float2 smem[SMEM];
-
-    const int tid = threadIdx.x;
-    const int lt = tid & (T - 1);
-    const int sub = tid / T;
-    const long long g = (long long)blockIdx.x * B + sub;
-    const bool active = g < a.batch;
-    float2 *L = smem + sub * LDSN;
-    float *Lf = reinterpret_cast<float *>(smem) + sub * LDSN;
     auto slot = [](int o) { return XM == XM_FLAT ? o : pad_slot<LOGE>(o); };
 
-    const unsigned flags = a.flags;
     const bool zp = (flags & RF_ZERO_PAD) != 0;
     const bool mul = (flags & RF_MUL) != 0;
     const bool cj = (flags & RF_CONJ2) != 0;
     const bool full = (flags & RF_FULL) != 0;
-
-    const float2 *X = a.in + g * a.idist;
-    const float2 *Y = a.in2 + g * a.idist2;
-    float2 *Xo = a.out + g * a.odist;
     // Element vp + sp of an operand: vp per lane, sp a compile-time slot offset.
     __amdgpu_buffer_rsrc_t rin, rin2, rout, rrtw;  // set and used only when BUF
     if constexpr (BUF) {
@@ -58,7 +49,7 @@ __global__ void __launch_bounds__(rv_WG(N, VAR)) fft_rows_kernel(RowArgs a) {
         rin = make_rsrc(X, active ? nin * 8u : 0u);
         rin2 = make_rsrc(Y, (active && mul) ? (KIND == KIND_C2R ? N + 1 : N) * 8u : 0u);
         rout = make_rsrc(Xo, active ? nout * 8u : 0u);
-        rrtw = make_rsrc(a.rtw, KIND == KIND_C2C ? 0u : N * 8u);
+        rrtw = make_rsrc(rtw, KIND == KIND_C2C ? 0u : N * 8u);
     }
     auto ld_in = [&](int vp, int sp) -> float2 {
         if constexpr (BUF) return bload(rin, (unsigned)vp * 8u, (unsigned)sp * 8u);
@@ -70,7 +61,7 @@ __global__ void __launch_bounds__(rv_WG(N, VAR)) fft_rows_kernel(RowArgs a) {
     };
     auto ld_rtw = [&](int vp, int sp) -> float2 {
         if constexpr (BUF) return bload(rrtw, (unsigned)vp * 8u, (unsigned)sp * 8u);
-        else return a.rtw[vp + sp];
+        else return rtw[vp + sp];
     };
     auto st_out = [&](int vp, int sp, float2 val) {
         if constexpr (BUF) bstore(rout, (unsigned)vp * 8u, (unsigned)sp * 8u, val);
@@ -95,8 +86,8 @@ __global__ void __launch_bounds__(rv_WG(N, VAR)) fft_rows_kernel(RowArgs a) {
                         ya = cconj(ya);
                         yb = cconj(yb);
                     }
-                    xa = cscale(cmul(xa, ya), a.norm);
-                    xb = cscale(cmul(xb, yb), a.norm);
+                    xa = cscale(cmul(xa, ya), norm);
+                    xb = cscale(cmul(xb, yb), norm);
                 }
                 const float2 bc = cconj(xb);
                 z = cadd(cadd(xa, bc), cmul(csub(xa, bc), ld_rtw(lt, T * c)));
@@ -112,7 +103,7 @@ __global__ void __launch_bounds__(rv_WG(N, VAR)) fft_rows_kernel(RowArgs a) {
                 if (BUF || active) {
                     float2 y = ld_in2(lt, T * c);
                     if (cj) y = cconj(y);
-                    z = cscale(cmul(ld_in(lt, T * c), y), a.norm);
+                    z = cscale(cmul(ld_in(lt, T * c), y), norm);
                 }
                 v[c] = z;
             });
@@ -126,9 +117,9 @@ __global__ void __launch_bounds__(rv_WG(N, VAR)) fft_rows_kernel(RowArgs a) {
 
     if constexpr (BUF) {
         const unsigned ntw = rv_NP(N, VAR) > 1 ? (unsigned)(N - E) : 0u;
-        stockham_passes<N, E, DIR, XM, TWM>(v, lt, TwBuffer{make_rsrc(a.tw, ntw * 8u)}, L, Lf, slot);
+        stockham_passes<N, E, DIR, XM, TWM>(v, lt, TwBuffer{make_rsrc(tw, ntw * 8u)}, L, Lf, slot);
     } else {
-        stockham_passes<N, E, DIR, XM, TWM>(v, lt, TwGlobal{a.tw}, L, Lf, slot);
+        stockham_passes<N, E, DIR, XM, TWM>(v, lt, TwGlobal{tw}, L, Lf, slot);
     }
 
     if constexpr (KIND == KIND_R2C) {
@@ -206,6 +197,23 @@ __global__ void __launch_bounds__(rv_WG(N, VAR)) fft_rows_kernel(RowArgs a) {
             });
         }
     }
+}
+
+
+template <int N, int DIR, int KIND, int VAR>
+__global__ void __launch_bounds__(rv_WG(N, VAR)) fft_rows_kernel(RowArgs a) {
+    constexpr int T = rv_T(N, VAR);
+    constexpr int B = rv_B(N, VAR);
+    constexpr int LDSN = rv_lds_per(N, VAR);
+    constexpr int SMEM = rv_xm(VAR) == XM_SPLIT ? (B * LDSN + 1) / 2 : B * LDSN;
+    __shared__ float2 smem[SMEM];
+    const int tid = threadIdx.x;
+    const int lt = tid & (T - 1);
+    const int sub = tid / T;
+    const long long g = (long long)blockIdx.x * B + sub;
+    row_transform<N, DIR, KIND, VAR>(a.in + g * a.idist, a.in2 + g * a.idist2, a.out + g * a.odist,
+                                     g < a.batch, lt, a.tw, a.rtw, a.flags, a.norm,
+                                     smem + sub * LDSN, reinterpret_cast<float *>(smem) + sub * LDSN);
 }
 
 // Table of the instantiated variants of one (direction, kind).
