@@ -56,6 +56,32 @@ int mufft_hip_execute_conv_output_batched(mufft_plan_conv *plan, void *output,
                                           unsigned howmany, size_t odist, size_t dist_first,
                                           size_t dist_second, void *stream);
 
+/* ---- Adjoints (for differentiation) ----
+ * The adjoint (conjugate transpose) of a plan's transform, on device memory,
+ * gradient convention dL/dRe + i dL/dIm:
+ *   c2c: the opposite-direction unnormalised c2c;
+ *   r2c: N/2+1 complex gradient bins (N with FULL_R2C) -> N real gradients;
+ *   c2r: N real gradients -> N/2+1 complex gradient bins, including the
+ *        reference's use of Im(X_0), Im(X_{N/2}) (kernel.c:36-45).
+ * Zero-padded plans return gradients for all N inputs; the caller keeps the
+ * first half.  Not available for zero-padded c2r plans.  Distances are in
+ * elements of each buffer's type, 0 = packed. */
+int mufft_hip_execute_adjoint_1d_batched(mufft_plan_1d *plan, void *output, const void *input,
+                                         unsigned howmany, size_t idist, size_t odist,
+                                         void *stream);
+/* output = T(norm * input * op(input2)), op = conj when conj2 != 0, T the plan's
+ * transform (adjoint = 0) or its adjoint (adjoint = 1); the multiply is fused
+ * into the transform's first pass.  idist2 = 0 uses one input2 block for every
+ * transform.  Needs a complex-input transform (c2c, c2r, adjoint of r2c). */
+int mufft_hip_execute_plan_1d_mul_batched(mufft_plan_1d *plan, void *output, const void *input,
+                                          const void *input2, unsigned howmany, size_t idist,
+                                          size_t idist2, size_t odist, float norm, int conj2,
+                                          int adjoint, void *stream);
+/* output[i] = norm * sum_b a_b[i] * op(b_b[i]), i < n (complex), op = conj when
+ * conj_b != 0: the batch reduction of a filter gradient.  Deterministic. */
+int mufft_hip_mul_reduce(void *output, const void *a, const void *b, unsigned n, unsigned howmany,
+                         size_t adist, size_t bdist, float norm, int conj_b, void *stream);
+
 /* Number of kernel launches one execution of the plan issues. */
 int mufft_hip_plan_1d_launches(const mufft_plan_1d *plan);
 int mufft_hip_plan_2d_launches(const mufft_plan_2d *plan);
@@ -114,6 +140,20 @@ int mufft_hip_execute_conv_batch_direct(mufft_hip_conv_batch *batch, void *strea
 int mufft_hip_conv_batch_status(mufft_hip_conv_batch *batch);
 int mufft_hip_conv_batch_info(const mufft_hip_conv_batch *batch, int *ntasks, int *grid);
 void mufft_hip_free_conv_batch(mufft_hip_conv_batch *batch);
+
+/* ---- Streaming overlap-add FIR filtering ----
+ * y = x * h for an unbounded stream x, hop N/2, FFT size N (power of two,
+ * 4..65536), real filter of ntaps <= N/2 taps (host or device pointer, copied).
+ * Each call filters nblocks * N/2 input samples into as many output samples
+ * (device buffers); the convolution tail carries over to the next call.
+ * Calls on one object must be ordered on one stream. */
+typedef struct mufft_hip_ola mufft_hip_ola;
+mufft_hip_ola *mufft_hip_create_ola(unsigned N, const float *taps, unsigned ntaps);
+int mufft_hip_execute_ola(mufft_hip_ola *ola, float *output, const float *input, unsigned nblocks,
+                          void *stream);
+/* Clears the carried tail (start of a new stream). */
+int mufft_hip_reset_ola(mufft_hip_ola *ola, void *stream);
+void mufft_hip_free_ola(mufft_hip_ola *ola);
 
 #ifdef __cplusplus
 }

@@ -23,6 +23,7 @@ from .api import (  # noqa: E402
     FORWARD,
     INVERSE,
     MufftError,
+    OverlapAdd,
     Plan1D,
     Plan2D,
     PlanConv,
@@ -36,5 +37,5 @@ __all__ = [
     "FLAG_ZERO_PAD_UPPER_HALF", "CONV_METHOD_FLAG_MONO_MONO", "CONV_METHOD_FLAG_STEREO_MONO",
     "CONV_METHOD_FLAG_ZERO_PAD_UPPER_HALF_FIRST", "CONV_METHOD_FLAG_ZERO_PAD_UPPER_HALF_SECOND",
     "CONV_BLOCK_FIRST", "CONV_BLOCK_SECOND", "MufftError", "Plan1D", "Plan2D", "PlanConv",
-    "device_count", "get_convolve_func", "twiddle_table", "ConvBatch",
+    "device_count", "get_convolve_func", "twiddle_table", "ConvBatch", "OverlapAdd",
 ]

@@ -57,6 +57,14 @@ SIGNATURES = [
     ("mufft_hip_autotune_1d", _i, [_u, _i, _u, ctypes.POINTER(ctypes.c_double)]),
     ("mufft_hip_tune_save", _i, [ctypes.c_char_p]),
     ("mufft_hip_tune_load", _i, [ctypes.c_char_p]),
+    ("mufft_hip_execute_adjoint_1d_batched", _i, [_vp, _vp, _vp, _u, _sz, _sz, _vp]),
+    ("mufft_hip_execute_plan_1d_mul_batched", _i,
+     [_vp, _vp, _vp, _vp, _u, _sz, _sz, _sz, _f, _i, _i, _vp]),
+    ("mufft_hip_mul_reduce", _i, [_vp, _vp, _vp, _u, _u, _sz, _sz, _f, _i, _vp]),
+    ("mufft_hip_create_ola", _vp, [_u, _vp, _u]),
+    ("mufft_hip_execute_ola", _i, [_vp, _vp, _vp, _u, _vp]),
+    ("mufft_hip_reset_ola", _i, [_vp, _vp]),
+    ("mufft_hip_free_ola", None, [_vp]),
     ("mufft_hip_create_conv_batch", _vp, [_vp, _u, _u]),
     ("mufft_hip_execute_conv_batch", _i, [_vp, _vp]),
     ("mufft_hip_execute_conv_batch_direct", _i, [_vp, _vp]),

@@ -100,6 +100,23 @@ class Plan1D:
             self._h, ptr(out), ptr(inp), howmany, idist, odist, stream_handle(stream))
         _check(rc, "mufft_hip_execute_plan_1d_batched")
 
+    def execute_adjoint_batched(self, out, inp, howmany: int, idist: int = 0, odist: int = 0,
+                                stream=None):
+        """Adjoint (conjugate transpose) of the plan's transform; see mufft_hip.h."""
+        rc = _lib.load().mufft_hip_execute_adjoint_1d_batched(
+            self._h, ptr(out), ptr(inp), howmany, idist, odist, stream_handle(stream))
+        _check(rc, "mufft_hip_execute_adjoint_1d_batched")
+
+    def execute_mul_batched(self, out, inp, inp2, howmany: int, idist: int = 0, idist2: int = 0,
+                            odist: int = 0, norm: float = 1.0, conj2: bool = False,
+                            adjoint: bool = False, stream=None):
+        """out = T(norm * inp * op(inp2)) with the multiply fused into the transform;
+        idist2 = 0 applies one inp2 block to every transform."""
+        rc = _lib.load().mufft_hip_execute_plan_1d_mul_batched(
+            self._h, ptr(out), ptr(inp), ptr(inp2), howmany, idist, idist2, odist,
+            ctypes.c_float(norm), int(conj2), int(adjoint), stream_handle(stream))
+        _check(rc, "mufft_hip_execute_plan_1d_mul_batched")
+
     def close(self):
         if getattr(self, "_h", None):
             _lib.load().mufft_free_plan_1d(self._h)
@@ -257,6 +274,45 @@ class ConvBatch:
     def close(self):
         if getattr(self, "_h", None):
             _lib.load().mufft_hip_free_conv_batch(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def mul_reduce(out, a, b, n: int, howmany: int, adist: int = 0, bdist: int = 0,
+               norm: float = 1.0, conj_b: bool = True, stream=None):
+    """out[i] = norm * sum_b a_b[i] * op(b_b[i]) on the device (filter-gradient reduction)."""
+    rc = _lib.load().mufft_hip_mul_reduce(ptr(out), ptr(a), ptr(b), n, howmany, adist, bdist,
+                                          ctypes.c_float(norm), int(conj_b), stream_handle(stream))
+    _check(rc, "mufft_hip_mul_reduce")
+
+
+class OverlapAdd:
+    """mufft_hip_ola: streaming FIR filtering by overlap-add (hop N/2)."""
+
+    def __init__(self, N: int, taps):
+        L = _lib.load()
+        n = int(taps.numel() if hasattr(taps, "numel") else len(taps))
+        h = L.mufft_hip_create_ola(N, ptr(taps), n)
+        if not h:
+            raise MufftError(f"mufft_hip_create_ola({N}) failed: {_lib.last_error()}")
+        self._h, self.N = h, N
+
+    def __call__(self, out, inp, nblocks: int, stream=None):
+        rc = _lib.load().mufft_hip_execute_ola(self._h, ptr(out), ptr(inp), nblocks,
+                                               stream_handle(stream))
+        _check(rc, "mufft_hip_execute_ola")
+
+    def reset(self, stream=None):
+        _check(_lib.load().mufft_hip_reset_ola(self._h, stream_handle(stream)), "mufft_hip_reset_ola")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.load().mufft_hip_free_ola(self._h)
             self._h = None
 
     def __del__(self):

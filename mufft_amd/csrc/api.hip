@@ -4,9 +4,9 @@
 // them one pass at a time on host memory (fft.c:325-435, fft.c:811-989).
 // Here a plan owns device twiddle tables (generated in FP64 on the GPU), a
 // private HIP stream and scratch space, and executing it enqueues one fused
-// kernel per transform axis whenever the axis fits a workgroup (1D/rows up
-// to 32768 complex points, columns up to 8192), or one generic Stockham pass
-// per launch beyond that.  Host buffers are staged through device memory so
+// kernel per transform axis whenever the axis fits one or two workgroups
+// (1D/rows up to 65536 complex points, columns up to 8192), or one generic
+// Stockham pass per launch beyond that.  Host buffers are staged through device memory so
 // the reference's host-memory calling convention keeps working.
 #include <hip/hip_runtime.h>
 
@@ -130,6 +130,21 @@ int build_resolve_table(int n, int direction, float2 **out, hipStream_t s) {
     return 0;
 }
 
+// Twiddles of a transform of n points split into n / n2 sub-transforms:
+// entry (k1 - 1) * n2 + j = exp(dir * 2 pi i * j * k1 / n).
+int build_split_table(int n, int n2, int direction, float2 **out, hipStream_t s) {
+    TwiddleArgs a{};
+    a.direction = direction;
+    a.mode = 3;
+    a.n = n;
+    a.p[0] = n2;
+    a.count = n - n2;
+    HIP_TRY(hipMalloc(out, sizeof(float2) * (size_t)a.count));
+    a.out = *out;
+    HIP_TRY(launch_twiddles(a, s));
+    return 0;
+}
+
 // Radix sequence (E, ..., E, r_last) of the fused kernels for E points per thread.
 std::vector<int> fused_sequence(int n, int E) {
     if (E > n) E = n;
@@ -179,6 +194,7 @@ struct Engine {
     float2 *tw_row = nullptr;     // sequence of the row variant
     float2 *tw_col = nullptr;     // sequence of the column kernel
     float2 *tw_gen = nullptr;     // sequence of the generic pass kernels
+    float2 *tw_split = nullptr;   // split variants: w_n^(dir*j*k1), 0 < k1 < N1, j < n/N1
     std::vector<int> gen_radix;
     bool row_fused() const { return logn <= ROW_MAX_LOG2; }
     bool col_fused() const { return logn <= COL_MAX_LOG2; }
@@ -190,7 +206,9 @@ struct Engine {
         direction = dir;
         if (rows && row_fused()) {
             row_var = force_var ? force_var : chosen_variant(logn, tune_kind(kind, dir));
-            if (build_stockham_table(n, dir, fused_sequence(n, rv_E(n, row_var)), &tw_row, s)) return -1;
+            const int n2 = rv_n(n, row_var);
+            if (build_stockham_table(n2, dir, fused_sequence(n2, rv_E(n, row_var)), &tw_row, s)) return -1;
+            if (rv_split(row_var) > 0 && build_split_table(n, n2, dir, &tw_split, s)) return -1;
         }
         if (cols && col_fused()) {
             if (build_stockham_table(n, dir, fused_sequence(n, col_E(n)), &tw_col, s)) return -1;
@@ -205,7 +223,8 @@ struct Engine {
         if (tw_row) (void)hipFree(tw_row);
         if (tw_col) (void)hipFree(tw_col);
         if (tw_gen) (void)hipFree(tw_gen);
-        tw_row = tw_col = tw_gen = nullptr;
+        if (tw_split) (void)hipFree(tw_split);
+        tw_row = tw_col = tw_gen = tw_split = nullptr;
     }
     int launches_generic() const { return (int)gen_radix.size(); }
 };
@@ -263,6 +282,7 @@ int run_rows(const Engine &e, int kind, const float2 *rtw, const float2 *in, lon
         a.out = out;
         a.tw = e.tw_row;
         a.rtw = rtw;
+        a.stw = e.tw_split;
         a.idist = idist;
         a.idist2 = idist2;
         a.odist = odist;
@@ -313,7 +333,7 @@ int run_rows(const Engine &e, int kind, const float2 *rtw, const float2 *in, lon
         r.idist = n;
         r.odist = odist;
         r.batch = batch;
-        r.flags = flags & RF_FULL;
+        r.flags = flags & (RF_FULL | RF_ADJ_C2R);
         HIP_TRY(launch_resolve_r2c(r, st));
         return 0;
     }
@@ -331,7 +351,7 @@ int run_rows(const Engine &e, int kind, const float2 *rtw, const float2 *in, lon
     // With RF_ZERO_PAD the resolve zeroes the upper half of its output: a
     // zero-padded c2r plan reads only the lower half of the resolved sequence
     // (radix2_half_p1 after resolve_c2r, fft.c:828-832).
-    r.flags = flags & (RF_MUL | RF_CONJ2 | RF_ZERO_PAD);
+    r.flags = flags & (RF_MUL | RF_CONJ2 | RF_ZERO_PAD | RF_ADJ_R2C);
     r.norm = norm;
     HIP_TRY(launch_resolve_c2r(r, st));
     return run_generic(e, s0.f2(), Lay{s0.f2(), n, 1}, Lay{out, odist, 1}, Lay{s1.f2(), n, 1}, 1,
@@ -390,6 +410,11 @@ struct mufft_plan_1d {
     DevBuf s0, s1, stage_in, stage_in2, stage_out;
     size_t in_bytes = 0, out_bytes = 0;   // per transform
     std::mutex lock;                       // serialises drop-in staging
+    // Adjoint transform, built on first use.
+    Engine adj;
+    float2 *adj_rtw = nullptr;
+    bool adj_ready = false;
+    DevBuf fold;
 };
 
 static void destroy_1d(mufft_plan_1d *p) {
@@ -397,6 +422,9 @@ static void destroy_1d(mufft_plan_1d *p) {
     if (p->stream) (void)hipStreamSynchronize(p->stream);
     p->eng.release();
     if (p->rtw) (void)hipFree(p->rtw);
+    p->adj.release();
+    if (p->adj_rtw) (void)hipFree(p->adj_rtw);
+    p->fold.release();
     p->s0.release();
     p->s1.release();
     p->stage_in.release();
@@ -468,6 +496,76 @@ static int exec_1d(mufft_plan_1d *p, void *out, const void *in, const void *in2,
     return run_rows(p->eng, p->kind, p->rtw, static_cast<const float2 *>(in), idist,
                     static_cast<const float2 *>(in2), idist2, static_cast<float2 *>(out), odist,
                     (int)howmany, flags, norm, p->s0, p->s1, st);
+}
+
+// Kind / direction / weighting of the adjoint of a plan's transform:
+// c2c -> opposite-direction c2c; r2c -> c2r with RF_ADJ_R2C; c2r -> r2c with RF_ADJ_C2R.
+static void adjoint_shape(const mufft_plan_1d *p, int *kind, int *dir, unsigned *flags) {
+    if (p->kind == KIND_C2C) {
+        *kind = KIND_C2C;
+        *dir = -p->direction;
+        *flags = 0;
+    } else if (p->kind == KIND_R2C) {
+        *kind = KIND_C2R;
+        *dir = MUFFT_INVERSE;
+        *flags = RF_ADJ_R2C;
+    } else {
+        *kind = KIND_R2C;
+        *dir = MUFFT_FORWARD;
+        *flags = RF_ADJ_C2R;
+    }
+}
+
+static int ensure_adjoint(mufft_plan_1d *p) {
+    if (p->adj_ready) return 0;
+    if (p->kind == KIND_C2R && p->zero_pad) {
+        set_err("no adjoint for zero-padded c2r plans");
+        return -1;
+    }
+    int kind, dir;
+    unsigned fl;
+    adjoint_shape(p, &kind, &dir, &fl);
+    if (p->adj.init((int)p->N, dir, kind, true, false, p->stream)) return -1;
+    if (kind != KIND_C2C && build_resolve_table((int)p->N, dir, &p->adj_rtw, p->stream)) return -1;
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    p->adj_ready = true;
+    return 0;
+}
+
+// Adjoint execution; distances in complex elements of the adjoint's input
+// (the forward output) and output (the forward input).
+static int exec_adjoint(mufft_plan_1d *p, void *out, const void *in, const void *in2,
+                        unsigned howmany, long long idist, long long idist2, long long odist,
+                        unsigned extra, float norm, hipStream_t st) {
+    if (ensure_adjoint(p)) return -1;
+    int kind, dir;
+    unsigned fl;
+    adjoint_shape(p, &kind, &dir, &fl);
+    const float2 *src = static_cast<const float2 *>(in);
+    long long sd = idist;
+    if (p->kind == KIND_R2C && p->full) {
+        if (extra & RF_MUL) {
+            set_err("fused multiply is not available for the adjoint of a full r2c plan");
+            return -1;
+        }
+        // Full spectrum (2N bins) folded onto bins 0..N first.
+        const long long nb = (long long)p->N + 1;
+        if (p->fold.ensure(sizeof(float2) * (size_t)nb * howmany)) return -1;
+        MulArgs m{};
+        m.a = src;
+        m.out = p->fold.f2();
+        m.n = 2LL * p->N;
+        m.adist = idist;
+        m.odist = nb;
+        m.batch = (int)howmany;
+        HIP_TRY(launch_fold(m, st));
+        src = p->fold.f2();
+        sd = nb;
+    }
+    const float2 *rtw = kind == KIND_C2C ? nullptr : p->adj_rtw;
+    return run_rows(p->adj, kind, rtw, src, sd, static_cast<const float2 *>(in2), idist2,
+                    static_cast<float2 *>(out), odist, (int)howmany, fl | extra, norm, p->s0, p->s1,
+                    st);
 }
 
 // Packed distances (complex elements) of a plan's input/output.
@@ -578,6 +676,83 @@ int mufft_hip_execute_plan_1d_batched(mufft_plan_1d *p, void *output, const void
     }
     return exec_1d(p, output, input, nullptr, howmany, id, 0, od, 0u, 1.0f,
                    resolve_stream(stream, p->stream));
+}
+
+// Converts API distances (elements of each buffer's type) to complex elements.
+static int api_dists(const mufft_plan_1d *p, bool adjoint, size_t idist, size_t odist,
+                     long long *id, long long *od) {
+    // Buffer types: forward c2c C->C, r2c R->C, c2r C->R; the adjoint swaps them.
+    const bool in_real = adjoint ? (p->kind == KIND_C2R) : (p->kind == KIND_R2C);
+    const bool out_real = adjoint ? (p->kind == KIND_R2C) : (p->kind == KIND_C2R);
+    const long long in_packed = adjoint ? (p->kind == KIND_C2R ? (long long)p->N : out_dist_c(p))
+                                        : in_dist_c(p);
+    const long long out_packed = adjoint ? (p->kind == KIND_R2C ? (long long)p->N
+                                            : (p->kind == KIND_C2R ? (long long)p->N + 1 : (long long)p->N))
+                                         : out_dist_c(p);
+    if ((in_real && (idist & 1)) || (out_real && (odist & 1))) {
+        set_err("real buffer distances must be even");
+        return -1;
+    }
+    *id = idist ? (long long)(in_real ? idist / 2 : idist) : in_packed;
+    *od = odist ? (long long)(out_real ? odist / 2 : odist) : out_packed;
+    return 0;
+}
+
+int mufft_hip_execute_adjoint_1d_batched(mufft_plan_1d *p, void *output, const void *input,
+                                         unsigned howmany, size_t idist, size_t odist,
+                                         void *stream) {
+    if (!p || !output || !input) {
+        set_err("null argument");
+        return -1;
+    }
+    long long id, od;
+    if (api_dists(p, true, idist, odist, &id, &od)) return -1;
+    return exec_adjoint(p, output, input, nullptr, howmany, id, 0, od, 0u, 1.0f,
+                        resolve_stream(stream, p->stream));
+}
+
+int mufft_hip_execute_plan_1d_mul_batched(mufft_plan_1d *p, void *output, const void *input,
+                                          const void *input2, unsigned howmany, size_t idist,
+                                          size_t idist2, size_t odist, float norm, int conj2,
+                                          int adjoint, void *stream) {
+    if (!p || !output || !input || !input2) {
+        set_err("null argument");
+        return -1;
+    }
+    // The multiply fuses into the prologue of kernels with complex input.
+    const int kind_run = adjoint ? (p->kind == KIND_C2C ? KIND_C2C : (p->kind == KIND_R2C ? KIND_C2R : KIND_R2C))
+                                 : p->kind;
+    if (kind_run == KIND_R2C) {
+        set_err("no fused multiply for transforms with real input");
+        return -1;
+    }
+    long long id, od;
+    if (api_dists(p, adjoint != 0, idist, odist, &id, &od)) return -1;
+    const unsigned extra = RF_MUL | (conj2 ? RF_CONJ2 : 0u);
+    hipStream_t st = resolve_stream(stream, p->stream);
+    if (adjoint)
+        return exec_adjoint(p, output, input, input2, howmany, id, (long long)idist2, od, extra, norm, st);
+    return exec_1d(p, output, input, input2, howmany, id, (long long)idist2, od, extra, norm, st);
+}
+
+int mufft_hip_mul_reduce(void *output, const void *a, const void *b, unsigned n, unsigned howmany,
+                         size_t adist, size_t bdist, float norm, int conj_b, void *stream) {
+    if (!output || !a || !b) {
+        set_err("null argument");
+        return -1;
+    }
+    MulArgs m{};
+    m.a = static_cast<const float2 *>(a);
+    m.b = static_cast<const float2 *>(b);
+    m.out = static_cast<float2 *>(output);
+    m.n = n;
+    m.adist = adist ? (long long)adist : n;
+    m.bdist = bdist ? (long long)bdist : n;
+    m.batch = (int)howmany;
+    m.norm = norm;
+    m.flags = conj_b ? RF_CONJ2 : 0u;
+    HIP_TRY(launch_mul_reduce(m, static_cast<hipStream_t>(stream)));
+    return 0;
 }
 
 int mufft_hip_plan_1d_launches(const mufft_plan_1d *p) {
@@ -1239,6 +1414,7 @@ int mufft_hip_autotune_1d(unsigned N, int kind, unsigned howmany, double *best_u
         a.out = dout.f2();
         a.tw = e.tw_row;
         a.rtw = rtw;
+        a.stw = e.tw_split;
         a.idist = idist;
         a.odist = odist;
         a.batch = (int)howmany;
@@ -1610,6 +1786,106 @@ int mufft_hip_conv_batch_info(const mufft_hip_conv_batch *b, int *ntasks, int *g
     if (!b) return -1;
     if (ntasks) *ntasks = (int)b->tasks.size();
     if (grid) *grid = b->grid;
+    return 0;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Streaming overlap-add FIR filtering (the filter spectrum is computed once,
+// as the reference's callers cache it, bench.c:251-257, fft.h:170-173).
+// ===========================================================================
+struct mufft_hip_ola {
+    unsigned N = 0;
+    mufft_plan_1d *fwd = nullptr;   // r2c, zero-padded input (hop N/2)
+    mufft_plan_1d *inv = nullptr;   // c2r with the fused filter multiply
+    DevBuf H, X, Y, tail[2];
+    int cur = 0;
+    size_t cap_blocks = 0;
+};
+
+extern "C" {
+
+void mufft_hip_free_ola(mufft_hip_ola *o) {
+    if (!o) return;
+    if (o->fwd) (void)hipStreamSynchronize(o->fwd->stream);
+    (void)hipDeviceSynchronize();
+    destroy_1d(o->fwd);
+    destroy_1d(o->inv);
+    o->H.release();
+    o->X.release();
+    o->Y.release();
+    o->tail[0].release();
+    o->tail[1].release();
+    delete o;
+}
+
+mufft_hip_ola *mufft_hip_create_ola(unsigned N, const float *taps, unsigned ntaps) {
+    if (!is_pow2(N) || N < 4 || N > 65536 || !taps || ntaps == 0 || ntaps > N / 2) {
+        set_err("invalid size or taps (N power of two in [4, 65536], 1 <= taps <= N/2)");
+        return nullptr;
+    }
+    mufft_hip_ola *o = new (std::nothrow) mufft_hip_ola;
+    if (!o) return nullptr;
+    o->N = N;
+    o->fwd = mufft_create_plan_1d_r2c(N, MUFFT_FLAG_ZERO_PAD_UPPER_HALF);
+    o->inv = mufft_create_plan_1d_c2r(N, 0);
+    const size_t h = N / 2;
+    if (!o->fwd || !o->inv || o->H.ensure((h + 1) * sizeof(float2)) ||
+        o->tail[0].ensure(h * sizeof(float)) || o->tail[1].ensure(h * sizeof(float))) {
+        mufft_hip_free_ola(o);
+        return nullptr;
+    }
+    std::vector<float> hp(h, 0.0f);
+    if (hipMemcpy(hp.data(), taps, ntaps * sizeof(float), hipMemcpyDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        memcpy(hp.data(), taps, ntaps * sizeof(float));  // plain host memory
+    }
+    DevBuf dh;
+    hipStream_t st = o->fwd->stream;
+    if (dh.ensure(h * sizeof(float)) ||
+        hipMemcpy(dh.p, hp.data(), h * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+        exec_1d(o->fwd, o->H.p, dh.p, nullptr, 1, (long long)h / 2, 0, (long long)h + 1, 0u, 1.0f, st) ||
+        hipMemsetAsync(o->tail[0].p, 0, h * sizeof(float), st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        dh.release();
+        if (!g_err[0]) set_err("filter transform failed");
+        mufft_hip_free_ola(o);
+        return nullptr;
+    }
+    dh.release();
+    return o;
+}
+
+int mufft_hip_reset_ola(mufft_hip_ola *o, void *stream) {
+    if (!o) return -1;
+    HIP_TRY(hipMemsetAsync(o->tail[o->cur].p, 0, o->N / 2 * sizeof(float),
+                           static_cast<hipStream_t>(stream)));
+    return 0;
+}
+
+int mufft_hip_execute_ola(mufft_hip_ola *o, float *output, const float *input, unsigned nblocks,
+                          void *stream) {
+    if (!o || !output || !input) {
+        set_err("null argument");
+        return -1;
+    }
+    if (nblocks == 0) return 0;
+    const long long n = o->N, h = n / 2;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (o->X.ensure(sizeof(float2) * (size_t)(h + 1) * nblocks) ||
+        o->Y.ensure(sizeof(float) * (size_t)n * nblocks))
+        return -1;
+    // 1. spectra of every hop (zero padded to N); block b starts at input + b*h.
+    if (exec_1d(o->fwd, o->X.p, input, nullptr, nblocks, h / 2, 0, h + 1, 0u, 1.0f, st)) return -1;
+    // 2. filtered blocks: c2r of X * H / N (H broadcast to every block).
+    if (exec_1d(o->inv, o->Y.p, o->X.p, o->H.p, nblocks, h + 1, 0, h, RF_MUL, 1.0f / (float)n, st))
+        return -1;
+    // 3. overlap-add with the tail of the previous call.
+    float *tail_in = static_cast<float *>(o->tail[o->cur].p);
+    float *tail_out = static_cast<float *>(o->tail[o->cur ^ 1].p);
+    HIP_TRY(launch_ola(static_cast<const float *>(o->Y.p), output, tail_in, tail_out, n, nblocks, st));
+    o->cur ^= 1;
     return 0;
 }
 

@@ -23,6 +23,9 @@ enum RowFlag : unsigned {
     RF_FULL = 1u << 1,       // r2c writes all N bins (MUFFT_FLAG_FULL_R2C)
     RF_MUL = 1u << 2,        // input = norm * in * in2 (fused convolution multiply)
     RF_CONJ2 = 1u << 3,      // with RF_MUL: use conj(in2) (correlation / adjoint)
+    // Adjoint weightings (gradient convention dL/dRe + i dL/dIm):
+    RF_ADJ_R2C = 1u << 4,    // c2r prologue: interior bins x 1/2, Im of DC and Nyquist dropped
+    RF_ADJ_C2R = 1u << 5,    // r2c epilogue: interior bins x 2, DC/Nyquist cross terms
 };
 
 MUFFT_HD constexpr int ilog2(long long n) { return n <= 1 ? 0 : 1 + ilog2(n >> 1); }
@@ -53,19 +56,29 @@ MUFFT_HD constexpr int row_lds_per(int N) { return N + N / row_E(N); }  // padde
 // 0 every twiddle from the table, 1 split product).  The planner's static
 // choice is rv_default(); the autotuner may pick any registered variant.
 // ---------------------------------------------------------------------------
+//
+// A variant may also split one transform over N1 = 2^s workgroups (bits 16-19
+// hold s; split_impl.h): each workgroup then runs an N/N1-point transform, and
+// E, T, the pass count and the LDS size below refer to that sub-transform.
 MUFFT_HD constexpr int rv_make(int E, int xm, int twm) { return E | (xm << 8) | (twm << 12); }
-MUFFT_HD constexpr int rv_E(int N, int var) { return (var & 0xff) >= N ? N : (var & 0xff); }
+MUFFT_HD constexpr int rv_make_split(int log2n1, int var) { return var | (log2n1 << 16); }
+MUFFT_HD constexpr int rv_split(int var) { return (var >> 16) & 0xf; }   // log2 N1
+MUFFT_HD constexpr int rv_sub(int var) { return var & 0xffff; }          // variant of the sub-transform
+MUFFT_HD constexpr int rv_n(int N, int var) { return N >> rv_split(var); }
+MUFFT_HD constexpr int rv_E(int N, int var) { return (var & 0xff) >= rv_n(N, var) ? rv_n(N, var) : (var & 0xff); }
 MUFFT_HD constexpr int rv_xm(int var) { return (var >> 8) & 0xf; }
 MUFFT_HD constexpr int rv_twm(int var) { return (var >> 12) & 0xf; }
-MUFFT_HD constexpr int rv_T(int N, int var) { return N / rv_E(N, var); }
+MUFFT_HD constexpr int rv_T(int N, int var) { return rv_n(N, var) / rv_E(N, var); }
 MUFFT_HD constexpr int rv_B(int N, int var) { return rv_T(N, var) >= 256 ? 1 : 256 / rv_T(N, var); }
 MUFFT_HD constexpr int rv_WG(int N, int var) { return rv_T(N, var) * rv_B(N, var); }
 MUFFT_HD constexpr int rv_NP(int N, int var) {
-    return rv_E(N, var) >= N ? 1 : (ilog2(N) + ilog2(rv_E(N, var)) - 1) / ilog2(rv_E(N, var));
+    return rv_E(N, var) >= rv_n(N, var)
+               ? 1
+               : (ilog2(rv_n(N, var)) + ilog2(rv_E(N, var)) - 1) / ilog2(rv_E(N, var));
 }
 // LDS slots per transform (points for complex modes, floats for the split mode).
 MUFFT_HD constexpr int rv_lds_per(int N, int var) {
-    return rv_xm(var) == 2 ? N : N + N / rv_E(N, var);
+    return rv_xm(var) == 2 ? rv_n(N, var) : rv_n(N, var) + rv_n(N, var) / rv_E(N, var);
 }
 MUFFT_HD constexpr long long rv_lds_bytes(int N, int var) {
     return (long long)rv_B(N, var) * rv_lds_per(N, var) * (rv_xm(var) == 1 ? 4 : 8);
@@ -79,7 +92,8 @@ MUFFT_HD constexpr int rv_default(int N) {
          : N == 1024 ? rv_make(32, 0, 1)
          : N <= 8192 ? rv_make(16, 0, 1)
          : N == 16384 ? rv_make(32, 0, 1)
-         : rv_make(32, 1, 1);
+         : N == 32768 ? rv_make_split(1, rv_make(32, 0, 1))
+         : rv_make_split(1, rv_make(32, 1, 1));
 }
 
 // Radix of pass s for a radix sequence that uses E for all but the last pass.
@@ -102,7 +116,8 @@ MUFFT_HD constexpr int col_WG(int L) { return col_T(L) * col_TX(L); }
 MUFFT_HD constexpr int col_NP(int L) { return row_NP(L); }
 MUFFT_HD constexpr int col_lds_rows(int L) { return L + L / col_E(L); }
 constexpr int COL_MAX_LOG2 = 13;   // fused column kernel covers L <= 8192
-constexpr int ROW_MAX_LOG2 = 15;   // fused row kernel covers N <= 32768
+constexpr int ROW_MAX_LOG2 = 16;   // fused row kernels (one or two workgroups per transform) cover N <= 65536
+constexpr int ROW_WG_MAX_LOG2 = 15;  // one workgroup per transform up to 32768 points
 
 // Twiddle table layout for a Stockham radix sequence r_0, r_1, ...: pass s >= 1
 // (stride p_s) owns (r_s - 1) * p_s entries starting at p_s - r_0; entry

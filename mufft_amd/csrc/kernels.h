@@ -16,6 +16,7 @@ struct RowArgs {
     float2 *out;         // c2c: N points; r2c: N+1 bins (2N with RF_FULL); c2r: 2N reals
     const float2 *tw;    // Stockham twiddles, layout of tw_offset()
     const float2 *rtw;   // real-transform resolve twiddles W[k], k < N
+    const float2 *stw;   // split variants: w_N^(dir*n*k1) at (k1-1)*(N/N1) + n, 0 < k1 < N1
     long long idist, idist2, odist;  // distance between transforms, in complex elements
     int batch;
     unsigned flags;      // RowFlag
@@ -77,7 +78,8 @@ struct TwiddleArgs {
     int count;
     // mode 0: Stockham table for a radix sequence (npass passes with radix[s], p[s]).
     // mode 1: real-transform resolve twiddles W[k] = (0, dir) * exp(i pi dir k / n).
-    // mode 2: plain table exp(i pi dir k / (n/2)) for k < count (four-step / adjoints).
+    // mode 2: plain table exp(i pi dir k / (n/2)) for k < count.
+    // mode 3: split twiddles exp(i pi dir (e mod p[0]) (e / p[0] + 1) / (n/2)), e < count.
     int mode;
     int r0;
     int npass;
@@ -96,6 +98,10 @@ hipError_t launch_pass(int radix, int direction, const PassArgs &a, hipStream_t 
 hipError_t launch_resolve_r2c(const ResolveArgs &a, hipStream_t s);
 hipError_t launch_resolve_c2r(const ResolveArgs &a, hipStream_t s);
 hipError_t launch_mul(const MulArgs &a, hipStream_t s);
+hipError_t launch_fold(const MulArgs &a, hipStream_t s);
+hipError_t launch_mul_reduce(const MulArgs &a, hipStream_t s);
+hipError_t launch_ola(const float *y, float *out, const float *tail_in, float *tail_out,
+                      long long n, long long nblocks, hipStream_t s);
 hipError_t launch_twiddles(const TwiddleArgs &a, hipStream_t s);
 
 

@@ -56,7 +56,9 @@ hipError_t launch_rows(int logN, int direction, int kind, int var, const RowArgs
     const void *k = row_tables().t[table][idx];
     if (a.batch <= 0) return hipSuccess;
     const int N = 1 << logN;
-    const long long nblocks = ((long long)a.batch + rv_B(N, var) - 1) / rv_B(N, var);
+    const long long nblocks = rv_split(var) > 0
+                                  ? (long long)a.batch << rv_split(var)
+                                  : ((long long)a.batch + rv_B(N, var) - 1) / rv_B(N, var);
     if (nblocks > 0x7fffffffLL) return hipErrorInvalidValue;
     RowArgs args = a;
     void *params[] = {&args};
@@ -144,15 +146,18 @@ __global__ void __launch_bounds__(256) resolve_r2c_kernel(ResolveArgs a) {
         const float2 *Z = a.in + g * a.idist;
         float2 *X = a.out + g * a.odist;
         const float2 za = Z[k];
+        const bool adj = (a.flags & RF_ADJ_C2R) != 0;
         if (k == 0) {
-            X[0] = make_float2(za.x + za.y, 0.f);
-            X[a.n] = make_float2(za.x - za.y, 0.f);
+            const float r0 = za.x + za.y, rn = za.x - za.y;
+            X[0] = make_float2(r0, adj ? -rn : 0.f);
+            X[a.n] = make_float2(rn, adj ? -r0 : 0.f);
         } else {
             const float2 zb = cconj(Z[a.n - k]);
             const float2 fe = cadd(za, zb);
             const float2 fo = cmul(a.rtw[k], csub(za, zb));
-            X[k] = cscale(cadd(fe, fo), 0.5f);
-            if (full) X[k + a.n] = cscale(csub(fe, fo), 0.5f);
+            const float sc = adj ? 1.0f : 0.5f;
+            X[k] = cscale(cadd(fe, fo), sc);
+            if (full) X[k + a.n] = cscale(csub(fe, fo), sc);
         }
     }
 }
@@ -182,6 +187,10 @@ __global__ void __launch_bounds__(256) resolve_c2r_kernel(ResolveArgs a) {
             xa = cscale(cmul(xa, ya), a.norm);
             xb = cscale(cmul(xb, yb), a.norm);
         }
+        if (a.flags & RF_ADJ_R2C) {
+            xa = k == 0 ? make_float2(xa.x, 0.f) : cscale(xa, 0.5f);
+            xb = k == 0 ? make_float2(xb.x, 0.f) : cscale(xb, 0.5f);
+        }
         const float2 bc = cconj(xb);
         a.out[g * a.odist + k] = cadd(cadd(xa, bc), cmul(csub(xa, bc), a.rtw[k]));
     }
@@ -198,6 +207,81 @@ __global__ void __launch_bounds__(256) mul_kernel(MulArgs a) {
         if (cj) y = cconj(y);
         a.out[g * a.odist + i] = cscale(cmul(a.a[g * a.adist + i], y), a.norm);
     }
+}
+
+// Folds a full n-bin spectrum onto bins 0..n/2 for the adjoint of a full r2c:
+// G_k = H_k + conj(H_{n-k}) for 0 < k < n/2, G_0 = H_0, G_{n/2} = H_{n/2}.
+__global__ void __launch_bounds__(256) fold_kernel(MulArgs a) {
+    const long long half = a.n / 2;
+    const long long total = (half + 1) * a.batch;
+    for (long long gid = (long long)blockIdx.x * 256 + threadIdx.x; gid < total;
+         gid += (long long)gridDim.x * 256) {
+        const long long k = gid % (half + 1);
+        const long long g = gid / (half + 1);
+        const float2 *H = a.a + g * a.adist;
+        float2 v = H[k];
+        if (k > 0 && k < half) v = cadd(v, cconj(H[a.n - k]));
+        a.out[g * a.odist + k] = v;
+    }
+}
+
+// out[i] = norm * sum_b a_b[i] * op(b_b[i]) (op = conj with RF_CONJ2): the
+// batch reduction of a filter gradient.  One thread per bin, the batch loop
+// in a fixed order (deterministic).
+__global__ void __launch_bounds__(256) mul_reduce_kernel(MulArgs a) {
+    const bool cj = (a.flags & RF_CONJ2) != 0;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < a.n;
+         i += (long long)gridDim.x * 256) {
+        float2 acc = make_float2(0.f, 0.f);
+        for (int g = 0; g < a.batch; ++g) {
+            float2 y = a.b[g * a.bdist + i];
+            if (cj) y = cconj(y);
+            acc = cadd(acc, cmul(a.a[g * a.adist + i], y));
+        }
+        a.out[i] = cscale(acc, a.norm);
+    }
+}
+
+// Overlap-add of nblocks filtered blocks of n reals at hop n/2:
+// out[b*h + i] = y_b[i] + (b ? y_{b-1}[h + i] : tail_in[i]); tail_out = y_last[h..n).
+__global__ void __launch_bounds__(256) ola_kernel(const float *__restrict__ y, float *__restrict__ out,
+                                                  const float *__restrict__ tail_in,
+                                                  float *__restrict__ tail_out, long long n,
+                                                  long long nblocks) {
+    const long long h = n / 2;
+    const long long total = h * nblocks;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < total + h;
+         i += (long long)gridDim.x * 256) {
+        if (i < total) {
+            const long long b = i / h, r = i % h;
+            out[i] = y[b * n + r] + (b > 0 ? y[(b - 1) * n + h + r] : tail_in[r]);
+        } else {
+            const long long r = i - total;
+            tail_out[r] = y[(nblocks - 1) * n + h + r];
+        }
+    }
+}
+
+hipError_t launch_ola(const float *y, float *out, const float *tail_in, float *tail_out,
+                      long long n, long long nblocks, hipStream_t s) {
+    const long long work = n / 2 * (nblocks + 1);
+    if (nblocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ola_kernel, dim3(grid_for(work, 256)), dim3(256), 0, s, y, out, tail_in,
+                       tail_out, n, nblocks);
+    return hipGetLastError();
+}
+
+hipError_t launch_fold(const MulArgs &a, hipStream_t s) {
+    const long long work = (a.n / 2 + 1) * a.batch;
+    if (work <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fold_kernel, dim3(grid_for(work, 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_mul_reduce(const MulArgs &a, hipStream_t s) {
+    if (a.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(mul_reduce_kernel, dim3(grid_for(a.n, 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
 }
 
 hipError_t launch_resolve_r2c(const ResolveArgs &a, hipStream_t s) {
@@ -249,8 +333,13 @@ __global__ void __launch_bounds__(256) twiddle_kernel(TwiddleArgs a) {
         const float zero = 0.0f;
         a.out[e] = make_float2(__fsub_rn(__fmul_rn(zero, c), __fmul_rn(d, sn)),
                                __fadd_rn(__fmul_rn(zero, sn), __fmul_rn(c, d)));
-    } else {
+    } else if (a.mode == 2) {
         const double phase = (pi * dir * (double)e) / (double)(a.n / 2);
+        a.out[e] = make_float2((float)cos(phase), (float)sin(phase));
+    } else {
+        const long long n = e % a.p[0];
+        const long long k1 = e / a.p[0] + 1;
+        const double phase = (pi * dir * (double)(n * k1)) / (double)(a.n / 2);
         a.out[e] = make_float2((float)cos(phase), (float)sin(phase));
     }
 }

@@ -27,6 +27,10 @@ constexpr int V32S = rv_make(32, 1, 0);
 constexpr int V32ST = rv_make(32, 1, 1);
 constexpr int V64ST = rv_make(64, 1, 1);
 constexpr int V64PT = rv_make(64, 0, 1);
+// Two workgroups per transform (split_impl.h), with the sub-transform's variant.
+constexpr int S2_16PT = rv_make_split(1, V16PT);
+constexpr int S2_32PT = rv_make_split(1, V32PT);
+constexpr int S2_32ST = rv_make_split(1, V32ST);
 
 constexpr RowVariantEntry kRowVariants[] = {
     {1, V16P}, {2, V16P}, {3, V16P}, {4, V16P},
@@ -40,7 +44,9 @@ constexpr RowVariantEntry kRowVariants[] = {
     {12, V16P}, {12, V16PT}, {12, V8P}, {12, V16S}, {12, V16F}, {12, V32PT},
     {13, V32P}, {13, V32PT}, {13, V16PT}, {13, V16S},
     {14, V32P}, {14, V32PT}, {14, V16PT}, {14, V64PT},
-    {15, V32S}, {15, V32ST}, {15, V64ST},
+    {14, S2_16PT},
+    {15, V32S}, {15, V32ST}, {15, V64ST}, {15, S2_32PT}, {15, S2_16PT}, {15, S2_32ST},
+    {16, S2_32ST},
 };
 constexpr int kNumRowVariants = sizeof(kRowVariants) / sizeof(kRowVariants[0]);
 

@@ -11,6 +11,7 @@
 
 #include "kernels.h"
 #include "row_variants.h"
+#include "split_impl.h"
 #include "stockham.h"
 
 namespace mufft {
@@ -41,6 +42,7 @@ __device__ __forceinline__ void row_transform(const float2 *X, const float2 *Y, 
     const bool mul = (flags & RF_MUL) != 0;
     const bool cj = (flags & RF_CONJ2) != 0;
     const bool full = (flags & RF_FULL) != 0;
+    const bool adj = (flags & (RF_ADJ_R2C | RF_ADJ_C2R)) != 0;
     // Element vp + sp of an operand: vp per lane, sp a compile-time slot offset.
     __amdgpu_buffer_rsrc_t rin, rin2, rout, rrtw;  // set and used only when BUF
     if constexpr (BUF) {
@@ -89,6 +91,12 @@ __device__ __forceinline__ void row_transform(const float2 *X, const float2 *Y, 
                     xa = cscale(cmul(xa, ya), norm);
                     xb = cscale(cmul(xb, yb), norm);
                 }
+                if (adj) {
+                    // Adjoint of r2c: x = Re(sum_{k<=N} G_k w^{-nk}) equals c2r of G with
+                    // interior bins halved and the imaginary parts of bins 0 and N dropped.
+                    xa = c == 0 && lt == 0 ? make_float2(xa.x, 0.f) : cscale(xa, 0.5f);
+                    xb = c == 0 && lt == 0 ? make_float2(xb.x, 0.f) : cscale(xb, 0.5f);
+                }
                 const float2 bc = cconj(xb);
                 z = cadd(cadd(xa, bc), cmul(csub(xa, bc), ld_rtw(lt, T * c)));
             }
@@ -125,16 +133,21 @@ __device__ __forceinline__ void row_transform(const float2 *X, const float2 *Y, 
     if constexpr (KIND == KIND_R2C) {
         // Post-pass resolve (kernel.c:47-82): bin k needs Z[k] and Z[N - k],
         // which another thread holds, so Z goes through LDS once more.
+        // With RF_ADJ_C2R the epilogue emits the adjoint of c2r (muFFT semantics,
+        // kernel.c:36-45): 2*R_k for interior bins, R_0 - i R_N at bin 0 and
+        // R_N - i R_0 at bin N, where R = r2c of the incoming gradient.
+        const float scale = adj ? 1.0f : 0.5f;
         auto emit = [&](int c_sp, float2 za, float2 zn, bool dc) {
             if (dc) {
-                st_out(0, 0, make_float2(za.x + za.y, 0.f));
-                st_out(0, N, make_float2(za.x - za.y, 0.f));
+                const float r0 = za.x + za.y, rn = za.x - za.y;
+                st_out(0, 0, make_float2(r0, adj ? -rn : 0.f));
+                st_out(0, N, make_float2(rn, adj ? -r0 : 0.f));
             } else {
                 const float2 zb = cconj(zn);
                 const float2 fe = cadd(za, zb);
                 const float2 fo = cmul(ld_rtw(lt, c_sp), csub(za, zb));
-                st_out(lt, c_sp, cscale(cadd(fe, fo), 0.5f));
-                if (full) st_out(lt, c_sp + N, cscale(csub(fe, fo), 0.5f));
+                st_out(lt, c_sp, cscale(cadd(fe, fo), scale));
+                if (full) st_out(lt, c_sp + N, cscale(csub(fe, fo), scale));
             }
         };
         if constexpr (!SPLIT) {
@@ -221,7 +234,13 @@ template <int DIR, int KIND, int I = 0>
 void fill_row_kernels(const void **out) {
     if constexpr (I < kNumRowVariants) {
         constexpr RowVariantEntry e = kRowVariants[I];
-        out[I] = reinterpret_cast<const void *>(&fft_rows_kernel<(1 << e.logN), DIR, KIND, e.var>);
+        if constexpr (rv_split(e.var) > 0) {
+            constexpr int s = rv_split(e.var);
+            out[I] = reinterpret_cast<const void *>(
+                &fft_split_kernel<(1 << (e.logN - s)), (1 << s), DIR, KIND, rv_sub(e.var)>);
+        } else {
+            out[I] = reinterpret_cast<const void *>(&fft_rows_kernel<(1 << e.logN), DIR, KIND, e.var>);
+        }
         fill_row_kernels<DIR, KIND, I + 1>(out);
     }
 }

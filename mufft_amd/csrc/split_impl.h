@@ -1,0 +1,195 @@
+// One transform of N = N1 * N2 complex points spread over N1 workgroups
+// (decimation in frequency by N1, then a fused N2-point transform).
+//
+// Workgroup k1 of a transform produces the output points X[N1*k2 + k1]:
+//   y[n]          = w_N^(dir*n*k1) * sum_{n1 < N1} x[n + N2*n1] * w_N1^(dir*n1*k1),   n < N2
+//   X[N1*k2 + k1] = sum_{n < N2} y[n] * w_N2^(dir*n*k2)
+// so each workgroup keeps the register and LDS footprint of an N2-point
+// transform (several workgroups per CU) while the whole transform still costs
+// one compulsory HBM read and write.  The N1 workgroups of a transform are
+// consecutive logical blocks on one XCD (xcd_block): the input each of them
+// reads in full is fetched from HBM once and served to the others from that
+// XCD's L2, and their interleaved output points fill the same L2 lines.
+//
+// The real transforms use N1 = 2.  r2c: the reference's post-pass resolve
+// (kernel.c:47-82) pairs Z[k] with Z[N - k]; with k = 2*k2 + k1 the partner
+// is 2*(N2 - k2 - k1) + k1, in the same workgroup.  c2r: the pre-pass resolve
+// (kernel.c:36-45) forms each Z[m] from X[m] and X[N - m] as the points are
+// loaded.  The flags are those of the single-workgroup kernel (rows_impl.h):
+// zero padding, fused multiply, full r2c spectrum and the adjoint weightings.
+#pragma once
+
+#include "kernels.h"
+#include "stockham.h"
+
+namespace mufft {
+
+template <int N2, int N1, int DIR, int KIND, int VAR>
+__global__ void __launch_bounds__(rv_T(N2, VAR)) fft_split_kernel(RowArgs a) {
+    constexpr int E = rv_E(N2, VAR);
+    constexpr int LOGE = ilog2(E);
+    constexpr int T = rv_T(N2, VAR);
+    constexpr int XM = rv_xm(VAR);
+    constexpr int TWM = rv_twm(VAR);
+    constexpr bool SPLIT = XM == XM_SPLIT;
+    constexpr int NT = N1 * N2;
+    constexpr int LDSN = rv_lds_per(N2, VAR);
+    static_assert(rv_B(N2, VAR) == 1, "one sub-transform per workgroup");
+    static_assert(N1 == 2 || N1 == 4, "split factor");
+    static_assert(KIND == KIND_C2C || N1 == 2, "real transforms split in two");
+    __shared__ float2 smem[SPLIT ? (LDSN + 1) / 2 : LDSN];
+    float2 *L = smem;
+    float *Lf = reinterpret_cast<float *>(smem);
+    auto slot = [](int o) { return XM == XM_FLAT ? o : pad_slot<LOGE>(o); };
+
+    const unsigned g = xcd_block(blockIdx.x, gridDim.x);
+    const long long t = g / N1;
+    const int k1 = (int)(g % N1);
+    const int lt = threadIdx.x;
+    const unsigned flags = a.flags;
+    const bool zp = (flags & RF_ZERO_PAD) != 0;
+    const bool mul = (flags & RF_MUL) != 0;
+    const bool cj = (flags & RF_CONJ2) != 0;
+    const bool full = (flags & RF_FULL) != 0;
+    const bool adj = (flags & (RF_ADJ_R2C | RF_ADJ_C2R)) != 0;
+    const float norm = a.norm;
+
+    const unsigned nin = KIND == KIND_C2R ? NT + 1 : ((zp && !mul) ? NT / 2 : NT);
+    const unsigned nout = KIND == KIND_R2C ? (full ? 2 * NT : NT + 1) : NT;
+    const __amdgpu_buffer_rsrc_t rin = make_rsrc(a.in + t * a.idist, nin * 8u);
+    const __amdgpu_buffer_rsrc_t rin2 =
+        make_rsrc(a.in2 + t * a.idist2, mul ? (KIND == KIND_C2R ? NT + 1 : NT) * 8u : 0u);
+    const __amdgpu_buffer_rsrc_t rout = make_rsrc(a.out + t * a.odist, nout * 8u);
+    const __amdgpu_buffer_rsrc_t rrtw = make_rsrc(a.rtw, KIND == KIND_C2C ? 0u : NT * 8u);
+    const __amdgpu_buffer_rsrc_t rstw = make_rsrc(a.stw, (N1 - 1) * N2 * 8u);
+    auto ld = [](__amdgpu_buffer_rsrc_t r, int vp, int sp) {
+        return bload(r, (unsigned)vp * 8u, (unsigned)sp * 8u);
+    };
+
+    // w_N1^(dir*n1*k1) is one of 1, i, -1, -i: as an exact complex factor.
+    auto rot = [&](int n1) {
+        const int q = (n1 * k1 * (4 / N1)) & 3;  // quarter turns
+        const float c = q == 0 ? 1.f : (q == 2 ? -1.f : 0.f);
+        const float s = q == 1 ? (float)DIR : (q == 3 ? -(float)DIR : 0.f);
+        return make_float2(c, s);
+    };
+
+    float2 v[E];
+    static_for<0, E>([&](auto C) {
+        constexpr int c = decltype(C)::value;
+        float2 acc = make_float2(0.f, 0.f);
+        static_for<0, N1>([&](auto M1) {
+            constexpr int n1 = decltype(M1)::value;
+            constexpr int sp = T * c + N2 * n1;  // point m = lt + sp
+            if (zp && n1 >= N1 / 2) return;
+            float2 z;
+            if constexpr (KIND == KIND_C2R) {
+                // X[NT - m] is element (T - lt) + (NT - T*(c+1) - N2*n1).
+                constexpr int spb = NT - T * (c + 1) - N2 * n1;
+                float2 xa = ld(rin, lt, sp);
+                float2 xb = ld(rin, T - lt, spb);
+                if (mul) {
+                    float2 ya = ld(rin2, lt, sp);
+                    float2 yb = ld(rin2, T - lt, spb);
+                    if (cj) {
+                        ya = cconj(ya);
+                        yb = cconj(yb);
+                    }
+                    xa = cscale(cmul(xa, ya), norm);
+                    xb = cscale(cmul(xb, yb), norm);
+                }
+                if (adj) {
+                    const bool m0 = sp == 0 && lt == 0;
+                    xa = m0 ? make_float2(xa.x, 0.f) : cscale(xa, 0.5f);
+                    xb = m0 ? make_float2(xb.x, 0.f) : cscale(xb, 0.5f);
+                }
+                const float2 bc = cconj(xb);
+                z = cadd(cadd(xa, bc), cmul(csub(xa, bc), ld(rrtw, lt, sp)));
+            } else if (KIND == KIND_C2C && mul) {
+                float2 y = ld(rin2, lt, sp);
+                if (cj) y = cconj(y);
+                z = cscale(cmul(ld(rin, lt, sp), y), norm);
+            } else {
+                z = ld(rin, lt, sp);
+            }
+            acc = n1 == 0 ? z : cadd(acc, cmul(z, rot(n1)));
+        });
+        if (k1 > 0) acc = cmul(acc, ld(rstw, lt, (k1 - 1) * N2 + T * c));
+        v[c] = acc;
+    });
+
+    stockham_passes<N2, E, DIR, XM, TWM>(v, lt, TwBuffer{make_rsrc(a.tw, (N2 - E) * 8u)}, L, Lf, slot);
+
+    // v[c] now holds X[N1*k2 + k1] for k2 = lt + T*c.
+    if constexpr (KIND != KIND_R2C) {
+        static_for<0, E>([&](auto C) {
+            constexpr int c = decltype(C)::value;
+            bstore(rout, (unsigned)(N1 * lt) * 8u, (unsigned)(N1 * T * c + k1) * 8u, v[c]);
+        });
+    } else {
+        // Bin k = 2*k2 + k1 from Z[k] and its partner Z[NT - k] = v at
+        // k2' = (N2 - k2 - k1) mod N2 (emit as in rows_impl.h).
+        const float scale = adj ? 1.0f : 0.5f;
+        auto emit = [&](int c_sp, float2 za, float2 zn, bool dc) {
+            const unsigned vo = (unsigned)(2 * lt) * 8u;
+            const unsigned so = (unsigned)(2 * c_sp + k1) * 8u;
+            if (dc) {
+                const float r0 = za.x + za.y, rn = za.x - za.y;
+                bstore(rout, 0u, 0u, make_float2(r0, adj ? -rn : 0.f));
+                bstore(rout, 0u, (unsigned)NT * 8u, make_float2(rn, adj ? -r0 : 0.f));
+            } else {
+                const float2 zb = cconj(zn);
+                const float2 fe = cadd(za, zb);
+                const float2 fo = cmul(bload(rrtw, vo, so), csub(za, zb));
+                bstore(rout, vo, so, cscale(cadd(fe, fo), scale));
+                if (full) bstore(rout, vo, so + (unsigned)NT * 8u, cscale(csub(fe, fo), scale));
+            }
+        };
+        if constexpr (!SPLIT) {
+            static_for<0, E>([&](auto C) {
+                constexpr int c = decltype(C)::value;
+                L[slot(lt + T * c)] = v[c];
+            });
+            __syncthreads();
+            static_for<0, E>([&](auto C) {
+                constexpr int c = decltype(C)::value;
+                const int k2 = lt + T * c;
+                const float2 zn = L[slot((N2 - k2 - k1) & (N2 - 1))];
+                emit(T * c, v[c], zn, c == 0 && lt == 0 && k1 == 0);
+            });
+        } else {
+            // Half the points at a time through the split-exchange LDS:
+            // round A publishes k2 >= N2/2 and resolves k2 < N2/2 (whose
+            // partners are all in the upper half, bin 0 aside); round B the
+            // other way round.  With k1 = 0, k2 = N2/2 is its own partner.
+            float2 *Lc = reinterpret_cast<float2 *>(Lf);
+            static_for<E / 2, E>([&](auto C) {
+                constexpr int c = decltype(C)::value;
+                Lc[slot(lt + T * c - N2 / 2)] = v[c];
+            });
+            __syncthreads();
+            static_for<0, E / 2>([&](auto C) {
+                constexpr int c = decltype(C)::value;
+                const int k2 = lt + T * c;
+                const bool dc = c == 0 && lt == 0 && k1 == 0;
+                const float2 zn = Lc[slot(dc ? 0 : N2 - k2 - k1 - N2 / 2)];
+                emit(T * c, v[c], dc ? v[c] : zn, dc);
+            });
+            __syncthreads();
+            static_for<0, E / 2>([&](auto C) {
+                constexpr int c = decltype(C)::value;
+                Lc[slot(lt + T * c)] = v[c];
+            });
+            __syncthreads();
+            static_for<E / 2, E>([&](auto C) {
+                constexpr int c = decltype(C)::value;
+                const int k2 = lt + T * c;
+                const bool self = c == E / 2 && lt == 0 && k1 == 0;
+                const float2 zn = Lc[slot(self ? 0 : N2 - k2 - k1)];
+                emit(T * c, v[c], self ? v[c] : zn, false);
+            });
+        }
+    }
+}
+
+}  // namespace mufft

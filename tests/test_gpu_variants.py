@@ -43,7 +43,7 @@ def _run(gpu, kind, N, x, batch):
     return from_dev(dy)
 
 
-@pytest.mark.parametrize("logn", range(1, 16))
+@pytest.mark.parametrize("logn", range(1, 17))
 @pytest.mark.parametrize("kind", [0, 1, 2, 3])
 def test_every_variant(gpu, logn, kind):
     from mufft_amd import _lib
