@@ -90,6 +90,159 @@ bool is_host_ptr(const void *ptr) {
              attr.type == hipMemoryTypeArray);
 }
 
+// Where a buffer lives: device memory (used in place), pinned host memory
+// (kernels may address it over PCIe through `dev`) or pageable host memory
+// (only the copy engines can reach it).
+enum PtrKind { PTR_DEVICE, PTR_PINNED, PTR_PAGEABLE };
+struct PtrInfo {
+    PtrKind kind;
+    void *dev;
+};
+PtrInfo classify_ptr(const void *ptr) {
+    hipPointerAttribute_t attr;
+    if (hipPointerGetAttributes(&attr, ptr) != hipSuccess) {
+        (void)hipGetLastError();
+        return {PTR_PAGEABLE, nullptr};
+    }
+    if (attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged ||
+        attr.type == hipMemoryTypeArray)
+        return {PTR_DEVICE, const_cast<void *>(ptr)};
+    if (attr.type == hipMemoryTypeHost && attr.devicePointer) {
+        // devicePointer is the mapping of the allocation's base address.
+        const char *host_base = static_cast<const char *>(attr.hostPointer ? attr.hostPointer : ptr);
+        return {PTR_PINNED, static_cast<char *>(attr.devicePointer) +
+                                (static_cast<const char *>(ptr) - host_base)};
+    }
+    return {PTR_PAGEABLE, nullptr};
+}
+
+// Kernel-addressable form of a buffer for the stream-ordered extensions that
+// do no staging: device pointers as they are, pinned host memory through its
+// device mapping; pageable host memory is refused (a kernel cannot reach it).
+int kernel_ptr(const void *ptr, const void **out) {
+    const PtrInfo pi = classify_ptr(ptr);
+    if (pi.kind == PTR_PAGEABLE) {
+        set_err("buffer must be device memory or pinned host memory (mufft_alloc)");
+        return -1;
+    }
+    *out = pi.dev;
+    return 0;
+}
+template <typename T>
+int kernel_ptrs(T **p) {
+    const void *q = nullptr;
+    if (kernel_ptr(*p, &q)) return -1;
+    *p = static_cast<T *>(const_cast<void *>(q));
+    return 0;
+}
+template <typename T, typename... R>
+int kernel_ptrs(T **p, R... rest) {
+    return kernel_ptrs(p) ? -1 : kernel_ptrs(rest...);
+}
+
+// Host-memory batches: the batch runs in chunks through two device staging
+// slots on three streams, so the upload of chunk i+1 and the download of
+// chunk i-1 overlap the transform of chunk i.  Device layouts mirror the host
+// layouts (same distances), so the transform runs unchanged on each chunk.
+struct HostPipe {
+    static constexpr size_t kChunkBytes = 32u << 20;
+    hipStream_t up = nullptr, down = nullptr;
+    hipEvent_t ev_up[2] = {}, ev_comp[2] = {}, ev_down[2] = {};
+    DevBuf in[2], out[2];
+    int init() {
+        if (up) return 0;
+        HIP_TRY(hipStreamCreateWithFlags(&up, hipStreamNonBlocking));
+        HIP_TRY(hipStreamCreateWithFlags(&down, hipStreamNonBlocking));
+        for (int i = 0; i < 2; ++i) {
+            HIP_TRY(hipEventCreateWithFlags(&ev_up[i], hipEventDisableTiming));
+            HIP_TRY(hipEventCreateWithFlags(&ev_comp[i], hipEventDisableTiming));
+            HIP_TRY(hipEventCreateWithFlags(&ev_down[i], hipEventDisableTiming));
+        }
+        return 0;
+    }
+    void release() {
+        if (up) (void)hipStreamSynchronize(up);
+        if (down) (void)hipStreamSynchronize(down);
+        for (int i = 0; i < 2; ++i) {
+            if (ev_up[i]) (void)hipEventDestroy(ev_up[i]);
+            if (ev_comp[i]) (void)hipEventDestroy(ev_comp[i]);
+            if (ev_down[i]) (void)hipEventDestroy(ev_down[i]);
+            ev_up[i] = ev_comp[i] = ev_down[i] = nullptr;
+            in[i].release();
+            out[i].release();
+        }
+        if (up) (void)hipStreamDestroy(up);
+        if (down) (void)hipStreamDestroy(down);
+        up = down = nullptr;
+    }
+};
+
+// One operand of a staged batch: unit u occupies [u*stride, u*stride + unit)
+// bytes from `host`; `preload` uploads an output operand before the
+// transform (for transforms that leave part of their output untouched).
+struct StagedOperand {
+    void *host;
+    size_t unit, stride;
+    bool staged;
+};
+
+// Runs run(dev_out, dev_in, count) over `howmany` units in chunks; returns
+// when the last chunk's output is in host memory.
+template <typename Run>
+int run_staged(HostPipe &pp, hipStream_t comp, StagedOperand o, StagedOperand i, bool preload_out,
+               unsigned howmany, Run run) {
+    if (pp.init()) return -1;
+    const size_t big = std::max(o.stride, i.stride);
+    const unsigned per = (unsigned)std::max<size_t>(1, HostPipe::kChunkBytes / std::max<size_t>(big, 1));
+    const unsigned nchunks = (howmany + per - 1) / per;
+    for (int sl = 0; sl < 2; ++sl) {
+        const size_t cnt = std::min<size_t>(per, howmany);
+        if (i.staged && pp.in[sl].ensure((cnt - 1) * i.stride + i.unit)) return -1;
+        if (o.staged && pp.out[sl].ensure((cnt - 1) * o.stride + o.unit)) return -1;
+    }
+    // Work queued on `comp` before this call comes first.
+    HIP_TRY(hipEventRecord(pp.ev_comp[0], comp));
+    HIP_TRY(hipStreamWaitEvent(pp.up, pp.ev_comp[0], 0));
+    HIP_TRY(hipStreamWaitEvent(pp.down, pp.ev_comp[0], 0));
+    for (unsigned c = 0; c < nchunks; ++c) {
+        const int sl = c & 1;
+        const unsigned u0 = c * per;
+        const unsigned cnt = std::min(per, howmany - u0);
+        const size_t in_bytes = (size_t)(cnt - 1) * i.stride + i.unit;
+        const size_t out_bytes = (size_t)(cnt - 1) * o.stride + o.unit;
+        char *hin = static_cast<char *>(i.host) + (size_t)u0 * i.stride;
+        char *hout = static_cast<char *>(o.host) + (size_t)u0 * o.stride;
+        const bool up_work = i.staged || (o.staged && preload_out);
+        if (c >= 2) {
+            // Slot reuse: chunk c-2 must have been transformed and downloaded.
+            HIP_TRY(hipStreamWaitEvent(pp.up, pp.ev_comp[sl], 0));
+            if (o.staged) HIP_TRY(hipStreamWaitEvent(pp.up, pp.ev_down[sl], 0));
+        }
+        if (i.staged)
+            HIP_TRY(hipMemcpyAsync(pp.in[sl].p, hin, in_bytes, hipMemcpyHostToDevice, pp.up));
+        if (o.staged && preload_out)
+            HIP_TRY(hipMemcpyAsync(pp.out[sl].p, hout, out_bytes, hipMemcpyHostToDevice, pp.up));
+        if (up_work) {
+            HIP_TRY(hipEventRecord(pp.ev_up[sl], pp.up));
+            HIP_TRY(hipStreamWaitEvent(comp, pp.ev_up[sl], 0));
+        }
+        if (c >= 2 && o.staged) HIP_TRY(hipStreamWaitEvent(comp, pp.ev_down[sl], 0));
+        void *dout = o.staged ? pp.out[sl].p : static_cast<void *>(hout);
+        const void *din = i.staged ? pp.in[sl].p : static_cast<const void *>(hin);
+        if (run(dout, din, cnt)) return -1;
+        HIP_TRY(hipEventRecord(pp.ev_comp[sl], comp));
+        if (o.staged) {
+            HIP_TRY(hipStreamWaitEvent(pp.down, pp.ev_comp[sl], 0));
+            HIP_TRY(hipMemcpyAsync(hout, pp.out[sl].p, out_bytes, hipMemcpyDeviceToHost, pp.down));
+            HIP_TRY(hipEventRecord(pp.ev_down[sl], pp.down));
+        }
+    }
+    HIP_TRY(hipStreamSynchronize(pp.down));
+    HIP_TRY(hipStreamSynchronize(pp.up));
+    HIP_TRY(hipStreamSynchronize(comp));
+    return 0;
+}
+
 // Stockham twiddle table for a radix sequence, generated on the GPU in FP64.
 int build_stockham_table(int n, int direction, const std::vector<int> &radix, float2 **out,
                          hipStream_t s) {
@@ -130,6 +283,19 @@ int build_resolve_table(int n, int direction, float2 **out, hipStream_t s) {
     return 0;
 }
 
+// exp(dir * 2 pi i * j / n) for j < count.
+int build_plain_table(int n, int count, int direction, float2 **out, hipStream_t s) {
+    TwiddleArgs a{};
+    a.direction = direction;
+    a.mode = 2;
+    a.n = n;
+    a.count = count;
+    HIP_TRY(hipMalloc(out, sizeof(float2) * (size_t)count));
+    a.out = *out;
+    HIP_TRY(launch_twiddles(a, s));
+    return 0;
+}
+
 // Twiddles of a transform of n points split into n / n2 sub-transforms:
 // entry (k1 - 1) * n2 + j = exp(dir * 2 pi i * j * k1 / n).
 int build_split_table(int n, int n2, int direction, float2 **out, hipStream_t s) {
@@ -145,7 +311,7 @@ int build_split_table(int n, int n2, int direction, float2 **out, hipStream_t s)
     return 0;
 }
 
-// Radix sequence (E, ..., E, r_last) of the fused kernels for E points per thread.
+// Radix sequence (R, ..., R, r_last) of the fused kernels with pass radix R.
 std::vector<int> fused_sequence(int n, int E) {
     if (E > n) E = n;
     const int NP = E >= n ? 1 : (ilog2(n) + ilog2(E) - 1) / ilog2(E);
@@ -195,9 +361,14 @@ struct Engine {
     float2 *tw_col = nullptr;     // sequence of the column kernel
     float2 *tw_gen = nullptr;     // sequence of the generic pass kernels
     float2 *tw_split = nullptr;   // split variants: w_n^(dir*j*k1), 0 < k1 < N1, j < n/N1
+    // Four-step rows (n = n1 * n2 beyond the fused sizes): column tables of
+    // both halves and the two-level twiddle tables.
+    int log_n1 = 0, log_n2 = 0, log_s = 0;
+    float2 *tw_fs1 = nullptr, *tw_fs2 = nullptr, *tw_coarse = nullptr, *tw_fine = nullptr;
     std::vector<int> gen_radix;
     bool row_fused() const { return logn <= ROW_MAX_LOG2; }
     bool col_fused() const { return logn <= COL_MAX_LOG2; }
+    bool four_step() const { return logn > ROW_MAX_LOG2 && logn <= 2 * COL_MAX_LOG2; }
 
     // rows / cols: which roles this engine serves; kind: KIND_* of the row role.
     int init(int n_, int dir, int kind, bool rows, bool cols, hipStream_t s, int force_var = 0) {
@@ -207,13 +378,24 @@ struct Engine {
         if (rows && row_fused()) {
             row_var = force_var ? force_var : chosen_variant(logn, tune_kind(kind, dir));
             const int n2 = rv_n(n, row_var);
-            if (build_stockham_table(n2, dir, fused_sequence(n2, rv_E(n, row_var)), &tw_row, s)) return -1;
+            if (build_stockham_table(n2, dir, fused_sequence(n2, rv_R(n, row_var)), &tw_row, s)) return -1;
             if (rv_split(row_var) > 0 && build_split_table(n, n2, dir, &tw_split, s)) return -1;
         }
         if (cols && col_fused()) {
             if (build_stockham_table(n, dir, fused_sequence(n, col_E(n)), &tw_col, s)) return -1;
         }
-        if ((rows && !row_fused()) || (cols && !col_fused())) {
+        if (rows && four_step()) {
+            log_n1 = logn / 2;
+            log_n2 = logn - log_n1;
+            log_s = log_n2;
+            const int n1 = 1 << log_n1, n2 = 1 << log_n2;
+            if (build_stockham_table(n1, dir, fused_sequence(n1, col_E(n1)), &tw_fs1, s) ||
+                build_stockham_table(n2, dir, fused_sequence(n2, col_E(n2)), &tw_fs2, s) ||
+                build_plain_table(n >> log_s, n >> log_s, dir, &tw_coarse, s) ||
+                build_plain_table(n, 1 << log_s, dir, &tw_fine, s))
+                return -1;
+        }
+        if ((rows && !row_fused() && !four_step()) || (cols && !col_fused())) {
             gen_radix = generic_sequence(n);
             if (build_stockham_table(n, dir, gen_radix, &tw_gen, s)) return -1;
         }
@@ -224,7 +406,10 @@ struct Engine {
         if (tw_col) (void)hipFree(tw_col);
         if (tw_gen) (void)hipFree(tw_gen);
         if (tw_split) (void)hipFree(tw_split);
+        for (float2 *t : {tw_fs1, tw_fs2, tw_coarse, tw_fine})
+            if (t) (void)hipFree(t);
         tw_row = tw_col = tw_gen = tw_split = nullptr;
+        tw_fs1 = tw_fs2 = tw_coarse = tw_fine = nullptr;
     }
     int launches_generic() const { return (int)gen_radix.size(); }
 };
@@ -265,6 +450,46 @@ int run_generic(const Engine &e, const float2 *src, Lay s_lay, Lay fin, Lay alt,
         cur_sy = d.sy;
         p *= e.gen_radix[s];
     }
+    return 0;
+}
+
+// c2c transforms beyond the fused row sizes, packed scratch `alt` of
+// batch * n points: four-step (two fused column launches) up to
+// 2^(2 * COL_MAX_LOG2) points, generic passes beyond.  src, dst and alt must
+// be distinct.
+int run_large(const Engine &e, const float2 *src, long long sd, float2 *dst, long long dd,
+              float2 *alt, int batch, bool zp, hipStream_t st) {
+    if (!e.four_step())
+        return run_generic(e, src, Lay{const_cast<float2 *>(src), sd, 1}, Lay{dst, dd, 1},
+                           Lay{alt, e.n, 1}, 1, batch, zp, st);
+    const int n1 = 1 << e.log_n1, n2 = 1 << e.log_n2;
+    FourStepArgs f{};
+    f.in = src;
+    f.out = alt;
+    f.tw = e.tw_fs1;
+    f.coarse = e.tw_coarse;
+    f.fine = e.tw_fine;
+    f.in_bs = sd;
+    f.out_bs = e.n;
+    f.n2 = n2;
+    f.ntiles = n2 / col_TX(n1);
+    f.batch = batch;
+    f.log2s = e.log_s;
+    f.flags = zp ? RF_ZERO_PAD : 0u;
+    HIP_TRY(launch_four_step_cols(e.log_n1, e.direction, f, st));
+    // Second half: the N2-point column transforms of the n2 x n1 matrix.
+    ColArgs c{};
+    c.in = alt;
+    c.out = dst;
+    c.tw = e.tw_fs2;
+    c.in_rs = n1;
+    c.out_rs = n1;
+    c.in_bs = e.n;
+    c.out_bs = dd;
+    c.ncols = n1;
+    c.ntiles = (n1 + col_TX(n2) - 1) / col_TX(n2);
+    c.batch = batch;
+    HIP_TRY(launch_cols(e.log_n2, e.direction, c, st));
     return 0;
 }
 
@@ -318,13 +543,10 @@ int run_rows(const Engine &e, int kind, const float2 *rtw, const float2 *in, lon
             sd = n;
             zp_pass = false;
         }
-        return run_generic(e, src, Lay{const_cast<float2 *>(src), sd, 1}, Lay{out, odist, 1},
-                           Lay{s1.f2(), n, 1}, 1, batch, zp_pass, st);
+        return run_large(e, src, sd, out, odist, s1.f2(), batch, zp_pass, st);
     }
     if (kind == KIND_R2C) {
-        if (run_generic(e, in, Lay{const_cast<float2 *>(in), idist, 1}, Lay{s0.f2(), n, 1},
-                        Lay{s1.f2(), n, 1}, 1, batch, zp, st))
-            return -1;
+        if (run_large(e, in, idist, s0.f2(), n, s1.f2(), batch, zp, st)) return -1;
         ResolveArgs r{};
         r.in = s0.f2();
         r.out = out;
@@ -354,8 +576,7 @@ int run_rows(const Engine &e, int kind, const float2 *rtw, const float2 *in, lon
     r.flags = flags & (RF_MUL | RF_CONJ2 | RF_ZERO_PAD | RF_ADJ_R2C);
     r.norm = norm;
     HIP_TRY(launch_resolve_c2r(r, st));
-    return run_generic(e, s0.f2(), Lay{s0.f2(), n, 1}, Lay{out, odist, 1}, Lay{s1.f2(), n, 1}, 1,
-                       batch, false, st);
+    return run_large(e, s0.f2(), n, out, odist, s1.f2(), batch, false, st);
 }
 
 // Column transforms of length e.n over `ncols` columns, `batch` images.
@@ -409,7 +630,8 @@ struct mufft_plan_1d {
     float2 *rtw = nullptr;
     DevBuf s0, s1, stage_in, stage_in2, stage_out;
     size_t in_bytes = 0, out_bytes = 0;   // per transform
-    std::mutex lock;                       // serialises drop-in staging
+    std::mutex lock;                       // serialises drop-in staging and host batches
+    HostPipe pipe;                         // host-memory batches
     // Adjoint transform, built on first use.
     Engine adj;
     float2 *adj_rtw = nullptr;
@@ -430,6 +652,7 @@ static void destroy_1d(mufft_plan_1d *p) {
     p->stage_in.release();
     p->stage_in2.release();
     p->stage_out.release();
+    p->pipe.release();
     if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
 }
@@ -587,8 +810,19 @@ static void exec_1d_dropin(mufft_plan_1d *p, void *out, const void *in, const vo
     if (dev_prev != p->device) (void)hipSetDevice(p->device);
     const void *din = in, *din2 = in2;
     void *dout = out;
-    const bool hin = is_host_ptr(in), hout = is_host_ptr(out);
-    const bool hin2 = in2 && is_host_ptr(in2);
+    const PtrInfo ci = classify_ptr(in), co = classify_ptr(out);
+    const PtrInfo ci2 = in2 ? classify_ptr(in2) : PtrInfo{PTR_DEVICE, const_cast<void *>(in2)};
+    // One fused launch touches each point once: pinned host buffers are then
+    // read and written in place over PCIe, with no staging copies.
+    const bool direct = p->eng.row_fused() && ci.kind != PTR_PAGEABLE && co.kind != PTR_PAGEABLE &&
+                        ci2.kind != PTR_PAGEABLE;
+    const bool hin = !direct && ci.kind != PTR_DEVICE, hout = !direct && co.kind != PTR_DEVICE;
+    const bool hin2 = !direct && in2 && ci2.kind != PTR_DEVICE;
+    if (direct) {
+        din = ci.dev;
+        din2 = ci2.dev;
+        dout = co.dev;
+    }
     int rc = 0;
     if (hin) {
         rc |= p->stage_in.ensure(p->in_bytes);
@@ -674,8 +908,18 @@ int mufft_hip_execute_plan_1d_batched(mufft_plan_1d *p, void *output, const void
     } else {
         od = odist ? (long long)odist : out_dist_c(p);
     }
-    return exec_1d(p, output, input, nullptr, howmany, id, 0, od, 0u, 1.0f,
-                   resolve_stream(stream, p->stream));
+    hipStream_t st = resolve_stream(stream, p->stream);
+    const PtrInfo ci = classify_ptr(input), co = classify_ptr(output);
+    if (ci.kind == PTR_DEVICE && co.kind == PTR_DEVICE)
+        return exec_1d(p, output, input, nullptr, howmany, id, 0, od, 0u, 1.0f, st);
+    // Host memory: chunked through device staging, copies overlapping the transforms.
+    std::lock_guard<std::mutex> g(p->lock);
+    const StagedOperand so{output, (size_t)out_dist_c(p) * 8u, (size_t)od * 8u, co.kind != PTR_DEVICE};
+    const StagedOperand si{const_cast<void *>(input), (size_t)in_dist_c(p) * 8u, (size_t)id * 8u,
+                           ci.kind != PTR_DEVICE};
+    return run_staged(p->pipe, st, so, si, false, howmany, [&](void *o, const void *i, unsigned n) {
+        return exec_1d(p, o, i, nullptr, n, id, 0, od, 0u, 1.0f, st);
+    });
 }
 
 // Converts API distances (elements of each buffer's type) to complex elements.
@@ -706,7 +950,7 @@ int mufft_hip_execute_adjoint_1d_batched(mufft_plan_1d *p, void *output, const v
         return -1;
     }
     long long id, od;
-    if (api_dists(p, true, idist, odist, &id, &od)) return -1;
+    if (api_dists(p, true, idist, odist, &id, &od) || kernel_ptrs(&output, &input)) return -1;
     return exec_adjoint(p, output, input, nullptr, howmany, id, 0, od, 0u, 1.0f,
                         resolve_stream(stream, p->stream));
 }
@@ -727,7 +971,8 @@ int mufft_hip_execute_plan_1d_mul_batched(mufft_plan_1d *p, void *output, const 
         return -1;
     }
     long long id, od;
-    if (api_dists(p, adjoint != 0, idist, odist, &id, &od)) return -1;
+    if (api_dists(p, adjoint != 0, idist, odist, &id, &od) || kernel_ptrs(&output, &input, &input2))
+        return -1;
     const unsigned extra = RF_MUL | (conj2 ? RF_CONJ2 : 0u);
     hipStream_t st = resolve_stream(stream, p->stream);
     if (adjoint)
@@ -741,6 +986,7 @@ int mufft_hip_mul_reduce(void *output, const void *a, const void *b, unsigned n,
         set_err("null argument");
         return -1;
     }
+    if (kernel_ptrs(&output, &a, &b)) return -1;
     MulArgs m{};
     m.a = static_cast<const float2 *>(a);
     m.b = static_cast<const float2 *>(b);
@@ -758,7 +1004,8 @@ int mufft_hip_mul_reduce(void *output, const void *a, const void *b, unsigned n,
 int mufft_hip_plan_1d_launches(const mufft_plan_1d *p) {
     if (!p) return -1;
     if (p->eng.row_fused()) return 1;
-    return p->eng.launches_generic() + (p->kind == KIND_C2C ? 0 : 1);
+    const int c2c = p->eng.four_step() ? 2 : p->eng.launches_generic();
+    return c2c + (p->kind == KIND_C2C ? 0 : 1);
 }
 
 }  // extern "C"
@@ -779,6 +1026,7 @@ struct mufft_plan_2d {
     DevBuf s0, s1, s2, alt, stage_in, stage_out;
     size_t in_bytes = 0, out_bytes = 0;
     std::mutex lock;
+    HostPipe pipe;
 };
 
 static void destroy_2d(mufft_plan_2d *p) {
@@ -793,6 +1041,7 @@ static void destroy_2d(mufft_plan_2d *p) {
     p->alt.release();
     p->stage_in.release();
     p->stage_out.release();
+    p->pipe.release();
     if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
 }
@@ -947,12 +1196,25 @@ int mufft_hip_execute_plan_2d_batched(mufft_plan_2d *p, void *output, const void
         set_err("null argument");
         return -1;
     }
-    return exec_2d(p, output, input, howmany, resolve_stream(stream, p->stream));
+    hipStream_t st = resolve_stream(stream, p->stream);
+    const PtrInfo ci = classify_ptr(input), co = classify_ptr(output);
+    if (ci.kind == PTR_DEVICE && co.kind == PTR_DEVICE) return exec_2d(p, output, input, howmany, st);
+    // Host memory: chunked through device staging, copies overlapping the
+    // transforms.  A padded r2c output keeps the columns the transform does
+    // not write, so those images are uploaded first.
+    std::lock_guard<std::mutex> g(p->lock);
+    const bool preload = p->kind == KIND_R2C && !p->full;
+    const StagedOperand so{output, p->out_bytes, p->out_bytes, co.kind != PTR_DEVICE};
+    const StagedOperand si{const_cast<void *>(input), p->in_bytes, p->in_bytes, ci.kind != PTR_DEVICE};
+    return run_staged(p->pipe, st, so, si, preload, howmany, [&](void *o, const void *i, unsigned n) {
+        return exec_2d(p, o, i, n, st);
+    });
 }
 
 int mufft_hip_plan_2d_launches(const mufft_plan_2d *p) {
     if (!p) return -1;
-    int rows = p->ex.row_fused() ? 1 : p->ex.launches_generic() + (p->kind == KIND_C2C ? 0 : 1);
+    int rows = p->ex.row_fused() ? 1
+             : (p->ex.four_step() ? 2 : p->ex.launches_generic()) + (p->kind == KIND_C2C ? 0 : 1);
     int cols = p->ey.col_fused() ? 1 : p->ey.launches_generic();
     return rows + cols;
 }
@@ -1073,6 +1335,7 @@ int mufft_hip_execute_conv_output_batched(mufft_plan_conv *plan, void *output,
     } else {
         od = odist ? (long long)odist : (long long)op->N;
     }
+    if (kernel_ptrs(&output, &first, &second)) return -1;
     return exec_1d(op, output, first, second, howmany, d1, d2, od, RF_MUL, plan->normalization,
                    resolve_stream(stream, op->stream));
 }
@@ -1215,7 +1478,7 @@ int mufft_hip_twiddle_table(unsigned N, int direction, float *host_out, unsigned
     }
     hipStream_t s = nullptr;
     HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    std::vector<int> seq = fused_sequence((int)N, rv_E((int)N, rv_default((int)N)));
+    std::vector<int> seq = fused_sequence((int)N, rv_R((int)N, rv_default((int)N)));
     float2 *t = nullptr;
     int size = seq.size() > 1 ? (int)N - seq[0] : 0;
     int rc = build_stockham_table((int)N, direction, seq, &t, s);
@@ -1871,6 +2134,7 @@ int mufft_hip_execute_ola(mufft_hip_ola *o, float *output, const float *input, u
         return -1;
     }
     if (nblocks == 0) return 0;
+    if (kernel_ptrs(&output, &input)) return -1;
     const long long n = o->N, h = n / 2;
     hipStream_t st = static_cast<hipStream_t>(stream);
     if (o->X.ensure(sizeof(float2) * (size_t)(h + 1) * nblocks) ||

@@ -50,6 +50,85 @@ __global__ void __launch_bounds__(col_WG(L_)) fft_cols_kernel(ColArgs a) {
     }
 }
 
+// Four-step first half (FourStepArgs): the column transforms above plus the
+// w_N^(dir*n2*k1) twiddle, stored transposed through LDS so that each column
+// of the tile leaves as one contiguous run of N1 points.
+template <int L_, int DIR>
+__global__ void __launch_bounds__(col_WG(L_)) fft_four_step_cols_kernel(FourStepArgs a) {
+    constexpr int E = col_E(L_);
+    constexpr int LOGE = ilog2(E);
+    constexpr int T = col_T(L_);
+    constexpr int TX = col_TX(L_);
+    constexpr int WG = col_WG(L_);
+    constexpr int ROWS = col_lds_rows(L_);
+    constexpr int LP = L_ + 1;   // row pitch of the transposed tile
+    static_assert(LP <= ROWS, "transposed tile fits the exchange LDS");
+    __shared__ float2 smem[ROWS * TX];
+
+    const int tid = threadIdx.x;
+    const int x = tid & (TX - 1);
+    const int t = tid / TX;
+    const unsigned bid = xcd_block(blockIdx.x, gridDim.x);
+    const int img = (int)(bid / (unsigned)a.ntiles);
+    const int tile = (int)(bid % (unsigned)a.ntiles);
+    const int col = tile * TX + x;   // n2; the host launches whole tiles only
+    const bool zp = (a.flags & RF_ZERO_PAD) != 0;
+    auto slot = [x](int o) { return pad_slot<LOGE>(o) * TX + x; };
+
+    float2 v[E];
+    const float2 *src = a.in + (long long)img * a.in_bs + col;
+    static_for<0, E>([&](auto C) {
+        constexpr int c = decltype(C)::value;
+        const int row = t + T * c;
+        v[c] = (zp && row >= L_ / 2) ? make_float2(0.f, 0.f) : src[(long long)row * a.n2];
+    });
+
+    stockham_passes<L_, E, DIR, XM_PAD, TW_FULL>(v, t, TwGlobal{a.tw}, smem, nullptr, slot);
+
+    const int smask = (1 << a.log2s) - 1;
+    static_for<0, E>([&](auto C) {
+        constexpr int c = decltype(C)::value;
+        const unsigned e = (unsigned)col * (unsigned)(t + T * c);   // < N
+        const float2 w = cmul(a.coarse[e >> a.log2s], a.fine[e & smask]);
+        smem[x * LP + t + T * c] = cmul(v[c], w);
+    });
+    __syncthreads();
+    float2 *dst = a.out + (long long)img * a.out_bs + (long long)tile * TX * L_;
+    static_for<0, E>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        const int i = tid + WG * j;   // (column, k1) = (i / L, i % L)
+        dst[i] = smem[(i / L_) * LP + (i % L_)];
+    });
+}
+
+template <int DIR>
+static const void *four_step_kernel_ptr(int logL) {
+    switch (logL) {
+    case 6: return (const void *)&fft_four_step_cols_kernel<64, DIR>;
+    case 7: return (const void *)&fft_four_step_cols_kernel<128, DIR>;
+    case 8: return (const void *)&fft_four_step_cols_kernel<256, DIR>;
+    case 9: return (const void *)&fft_four_step_cols_kernel<512, DIR>;
+    case 10: return (const void *)&fft_four_step_cols_kernel<1024, DIR>;
+    case 11: return (const void *)&fft_four_step_cols_kernel<2048, DIR>;
+    case 12: return (const void *)&fft_four_step_cols_kernel<4096, DIR>;
+    case 13: return (const void *)&fft_four_step_cols_kernel<8192, DIR>;
+    default: return nullptr;
+    }
+}
+
+hipError_t launch_four_step_cols(int logN1, int direction, const FourStepArgs &a, hipStream_t s) {
+    const void *k = direction < 0 ? four_step_kernel_ptr<-1>(logN1) : four_step_kernel_ptr<1>(logN1);
+    if (!k) return hipErrorInvalidValue;
+    const int L = 1 << logN1;
+    if (a.n2 % col_TX(L) != 0 || a.ntiles != a.n2 / col_TX(L)) return hipErrorInvalidValue;
+    const long long nblocks = (long long)a.ntiles * a.batch;
+    if (nblocks <= 0) return hipSuccess;
+    if (nblocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    FourStepArgs args = a;
+    void *params[] = {&args};
+    return hipLaunchKernel(k, dim3((unsigned)nblocks), dim3(col_WG(L)), params, 0, s);
+}
+
 template <int DIR>
 static const void *col_kernel_ptr(int logL) {
     switch (logL) {

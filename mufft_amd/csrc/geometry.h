@@ -61,9 +61,18 @@ MUFFT_HD constexpr int row_lds_per(int N) { return N + N / row_E(N); }  // padde
 // hold s; split_impl.h): each workgroup then runs an N/N1-point transform, and
 // E, T, the pass count and the LDS size below refer to that sub-transform.
 MUFFT_HD constexpr int rv_make(int E, int xm, int twm) { return E | (xm << 8) | (twm << 12); }
+//
+// Bits 20-23 ask for that many workgroups per CU (0: no request): the kernel
+// is then compiled with a register budget that lets them co-reside.
 MUFFT_HD constexpr int rv_make_split(int log2n1, int var) { return var | (log2n1 << 16); }
+MUFFT_HD constexpr int rv_with_occ(int var, int wgs_per_cu) { return var | (wgs_per_cu << 20); }
+// Bits 24-27: log2 of the radix of every pass but the last (0: radix E).  A
+// radix below E gives each thread E / radix butterflies per pass: more passes,
+// fewer registers live inside one butterfly.
+MUFFT_HD constexpr int rv_with_radix(int var, int log2r) { return var | (log2r << 24); }
 MUFFT_HD constexpr int rv_split(int var) { return (var >> 16) & 0xf; }   // log2 N1
-MUFFT_HD constexpr int rv_sub(int var) { return var & 0xffff; }          // variant of the sub-transform
+MUFFT_HD constexpr int rv_occ(int var) { return (var >> 20) & 0xf; }
+MUFFT_HD constexpr int rv_sub(int var) { return var & ~(0xf << 16); }    // variant of the sub-transform
 MUFFT_HD constexpr int rv_n(int N, int var) { return N >> rv_split(var); }
 MUFFT_HD constexpr int rv_E(int N, int var) { return (var & 0xff) >= rv_n(N, var) ? rv_n(N, var) : (var & 0xff); }
 MUFFT_HD constexpr int rv_xm(int var) { return (var >> 8) & 0xf; }
@@ -71,10 +80,18 @@ MUFFT_HD constexpr int rv_twm(int var) { return (var >> 12) & 0xf; }
 MUFFT_HD constexpr int rv_T(int N, int var) { return rv_n(N, var) / rv_E(N, var); }
 MUFFT_HD constexpr int rv_B(int N, int var) { return rv_T(N, var) >= 256 ? 1 : 256 / rv_T(N, var); }
 MUFFT_HD constexpr int rv_WG(int N, int var) { return rv_T(N, var) * rv_B(N, var); }
+// Minimum waves per SIMD for the register budget (4 SIMDs per CU, 64-wide waves).
+MUFFT_HD constexpr int rv_waves_per_simd(int N, int var) {
+    return rv_occ(var) == 0 ? 1 : (rv_occ(var) * rv_WG(N, var) / 256 > 1 ? rv_occ(var) * rv_WG(N, var) / 256 : 1);
+}
+MUFFT_HD constexpr int rv_R(int N, int var) {
+    return ((var >> 24) & 0xf) == 0 ? rv_E(N, var)
+         : ((1 << ((var >> 24) & 0xf)) < rv_E(N, var) ? (1 << ((var >> 24) & 0xf)) : rv_E(N, var));
+}
 MUFFT_HD constexpr int rv_NP(int N, int var) {
-    return rv_E(N, var) >= rv_n(N, var)
+    return rv_R(N, var) >= rv_n(N, var)
                ? 1
-               : (ilog2(rv_n(N, var)) + ilog2(rv_E(N, var)) - 1) / ilog2(rv_E(N, var));
+               : (ilog2(rv_n(N, var)) + ilog2(rv_R(N, var)) - 1) / ilog2(rv_R(N, var));
 }
 // LDS slots per transform (points for complex modes, floats for the split mode).
 MUFFT_HD constexpr int rv_lds_per(int N, int var) {

@@ -35,6 +35,25 @@ struct ColArgs {
     int batch;                 // images
 };
 
+// First half of a four-step transform of N = N1 * N2 points (n = n1*N2 + n2):
+// the N1-point transforms down the columns of the N1 x N2 matrix, times
+// w_N^(dir*n2*k1), written transposed (element (k1, n2) to n2*N1 + k1) so the
+// second half is a plain column transform of length N2 over N1 columns.
+// w_N^e = coarse[e >> log2S] * fine[e & (S - 1)].
+struct FourStepArgs {
+    const float2 *in;
+    float2 *out;
+    const float2 *tw;      // Stockham table of the N1-point column kernel
+    const float2 *coarse;  // w_N^(dir*j*S), j < N/S
+    const float2 *fine;    // w_N^(dir*j), j < S
+    long long in_bs, out_bs;  // transform distance (complex elements)
+    int n2;                // columns = input row stride
+    int ntiles;            // column tiles per transform
+    int batch;
+    int log2s;
+    unsigned flags;        // RF_ZERO_PAD: rows n1 >= N1/2 are zero
+};
+
 // One Stockham pass of radix R over global memory (fallback for any size).
 struct PassArgs {
     const float2 *in;
@@ -100,6 +119,7 @@ hipError_t launch_resolve_c2r(const ResolveArgs &a, hipStream_t s);
 hipError_t launch_mul(const MulArgs &a, hipStream_t s);
 hipError_t launch_fold(const MulArgs &a, hipStream_t s);
 hipError_t launch_mul_reduce(const MulArgs &a, hipStream_t s);
+hipError_t launch_four_step_cols(int logN1, int direction, const FourStepArgs &a, hipStream_t s);
 hipError_t launch_ola(const float *y, float *out, const float *tail_in, float *tail_out,
                       long long n, long long nblocks, hipStream_t s);
 hipError_t launch_twiddles(const TwiddleArgs &a, hipStream_t s);

@@ -34,6 +34,7 @@ __device__ __forceinline__ void row_transform(const float2 *X, const float2 *Y, 
     constexpr int T = rv_T(N, VAR);
     constexpr int XM = rv_xm(VAR);
     constexpr int TWM = rv_twm(VAR);
+    constexpr int RAD = rv_R(N, VAR);
     constexpr bool SPLIT = XM == XM_SPLIT;
     constexpr bool BUF = T >= 64;
     auto slot = [](int o) { return XM == XM_FLAT ? o : pad_slot<LOGE>(o); };
@@ -124,83 +125,53 @@ __device__ __forceinline__ void row_transform(const float2 *X, const float2 *Y, 
     }
 
     if constexpr (BUF) {
-        const unsigned ntw = rv_NP(N, VAR) > 1 ? (unsigned)(N - E) : 0u;
-        stockham_passes<N, E, DIR, XM, TWM>(v, lt, TwBuffer{make_rsrc(tw, ntw * 8u)}, L, Lf, slot);
+        const unsigned ntw = rv_NP(N, VAR) > 1 ? (unsigned)(N - RAD) : 0u;
+        stockham_passes<N, E, DIR, XM, TWM, RAD>(v, lt, TwBuffer{make_rsrc(tw, ntw * 8u)}, L, Lf, slot);
     } else {
-        stockham_passes<N, E, DIR, XM, TWM>(v, lt, TwGlobal{tw}, L, Lf, slot);
+        stockham_passes<N, E, DIR, XM, TWM, RAD>(v, lt, TwGlobal{tw}, L, Lf, slot);
     }
 
     if constexpr (KIND == KIND_R2C) {
-        // Post-pass resolve (kernel.c:47-82): bin k needs Z[k] and Z[N - k],
-        // which another thread holds, so Z goes through LDS once more.
-        // With RF_ADJ_C2R the epilogue emits the adjoint of c2r (muFFT semantics,
-        // kernel.c:36-45): 2*R_k for interior bins, R_0 - i R_N at bin 0 and
-        // R_N - i R_0 at bin N, where R = r2c of the incoming gradient.
+        // Post-pass resolve (kernel.c:47-82).  Bins k and N - k both come from
+        // the pair Z[k], Z[N - k]: the upper half of Z (k >= N/2) goes through
+        // LDS once and the owner of each lower-half Z[k] emits both bins, so
+        // the upper-half registers die at the LDS store.  With RF_ADJ_C2R the
+        // epilogue emits the adjoint of c2r (muFFT semantics, kernel.c:36-45):
+        // 2*R_k for interior bins, R_0 - i R_N at bin 0 and R_N - i R_0 at bin N,
+        // where R = r2c of the incoming gradient.
         const float scale = adj ? 1.0f : 0.5f;
-        auto emit = [&](int c_sp, float2 za, float2 zn, bool dc) {
-            if (dc) {
-                const float r0 = za.x + za.y, rn = za.x - za.y;
-                st_out(0, 0, make_float2(r0, adj ? -rn : 0.f));
-                st_out(0, N, make_float2(rn, adj ? -r0 : 0.f));
-            } else {
-                const float2 zb = cconj(zn);
-                const float2 fe = cadd(za, zb);
-                const float2 fo = cmul(ld_rtw(lt, c_sp), csub(za, zb));
-                st_out(lt, c_sp, cscale(cadd(fe, fo), scale));
-                if (full) st_out(lt, c_sp + N, cscale(csub(fe, fo), scale));
-            }
+        // Bin (vp + sp) from za = Z[k] and zn = Z[N - k].
+        auto emit = [&](int vp, int sp, float2 za, float2 zn) {
+            const float2 zb = cconj(zn);
+            const float2 fe = cadd(za, zb);
+            const float2 fo = cmul(ld_rtw(vp, sp), csub(za, zb));
+            st_out(vp, sp, cscale(cadd(fe, fo), scale));
+            if (full) st_out(vp, sp + N, cscale(csub(fe, fo), scale));
         };
-        if constexpr (!SPLIT) {
-            static_for<0, E>([&](auto C) {
-                constexpr int c = decltype(C)::value;
-                L[slot(lt + T * c)] = v[c];
-            });
-            __syncthreads();
-            if (BUF || active) {
-                static_for<0, E>([&](auto C) {
-                    constexpr int c = decltype(C)::value;
-                    const int k = lt + T * c;
-                    emit(T * c, v[c], L[slot((N - k) & (N - 1))], c == 0 && lt == 0);
-                });
-            }
-        } else {
-            // The LDS of the split exchange holds half the spectrum as complex
-            // points.  Round A publishes the upper half (slots c >= E/2) and
-            // resolves the lower half; round B the other way round.  Partners
-            // are consumed as they are read, so no partner array stays live.
-            // Z[N/2] is its own partner (lt == 0, c == E/2).
-            float2 *Lc = reinterpret_cast<float2 *>(Lf);
-            static_for<E / 2, E>([&](auto C) {
-                constexpr int c = decltype(C)::value;
-                Lc[slot(lt + T * c - N / 2)] = v[c];
-            });
-            __syncthreads();
-            if (BUF || active) {
-                static_for<0, E / 2>([&](auto C) {
-                    constexpr int c = decltype(C)::value;
-                    const int k = lt + T * c;
-                    const bool dc = c == 0 && lt == 0;
-                    // Select values, not references: a select between a pointer
-                    // into v[] and an LDS pointer would force v[] into scratch.
-                    const float2 zn = Lc[slot(dc ? 0 : N - k - N / 2)];
-                    emit(T * c, v[c], dc ? v[c] : zn, dc);
-                });
-            }
-            __syncthreads();
+        float2 *Lc = SPLIT ? reinterpret_cast<float2 *>(Lf) : L;
+        static_for<E / 2, E>([&](auto C) {
+            constexpr int c = decltype(C)::value;
+            Lc[slot(lt + T * c - N / 2)] = v[c];
+        });
+        __syncthreads();
+        if (BUF || active) {
             static_for<0, E / 2>([&](auto C) {
                 constexpr int c = decltype(C)::value;
-                Lc[slot(lt + T * c)] = v[c];
+                if (c == 0 && lt == 0) {
+                    // Bins 0 and N from Z[0]; bin N/2 from Z[N/2], its own partner.
+                    const float2 z0 = v[0];
+                    const float r0 = z0.x + z0.y, rn = z0.x - z0.y;
+                    st_out(0, 0, make_float2(r0, adj ? -rn : 0.f));
+                    st_out(0, N, make_float2(rn, adj ? -r0 : 0.f));
+                    const float2 zh = Lc[slot(0)];
+                    emit(0, N / 2, zh, zh);
+                } else {
+                    // Z[N - k] for k = lt + T*c is upper-half point N/2 - k.
+                    const float2 zn = Lc[slot(N / 2 - lt - T * c)];
+                    emit(lt, T * c, v[c], zn);
+                    emit(T - lt, T * (E - 1 - c), zn, v[c]);
+                }
             });
-            __syncthreads();
-            if (BUF || active) {
-                static_for<E / 2, E>([&](auto C) {
-                    constexpr int c = decltype(C)::value;
-                    const int k = lt + T * c;
-                    const bool self = c == E / 2 && lt == 0;
-                    const float2 zn = Lc[slot(self ? 0 : N - k)];
-                    emit(T * c, v[c], self ? v[c] : zn, false);
-                });
-            }
         }
     } else {
         if (BUF || active) {
@@ -214,7 +185,7 @@ __device__ __forceinline__ void row_transform(const float2 *X, const float2 *Y, 
 
 
 template <int N, int DIR, int KIND, int VAR>
-__global__ void __launch_bounds__(rv_WG(N, VAR)) fft_rows_kernel(RowArgs a) {
+__global__ void __launch_bounds__(rv_WG(N, VAR), rv_waves_per_simd(N, VAR)) fft_rows_kernel(RowArgs a) {
     constexpr int T = rv_T(N, VAR);
     constexpr int B = rv_B(N, VAR);
     constexpr int LDSN = rv_lds_per(N, VAR);

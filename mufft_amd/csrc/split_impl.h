@@ -25,7 +25,7 @@
 namespace mufft {
 
 template <int N2, int N1, int DIR, int KIND, int VAR>
-__global__ void __launch_bounds__(rv_T(N2, VAR)) fft_split_kernel(RowArgs a) {
+__global__ void __launch_bounds__(rv_T(N2, VAR), rv_waves_per_simd(N2, VAR)) fft_split_kernel(RowArgs a) {
     constexpr int E = rv_E(N2, VAR);
     constexpr int LOGE = ilog2(E);
     constexpr int T = rv_T(N2, VAR);
@@ -118,7 +118,9 @@ __global__ void __launch_bounds__(rv_T(N2, VAR)) fft_split_kernel(RowArgs a) {
         v[c] = acc;
     });
 
-    stockham_passes<N2, E, DIR, XM, TWM>(v, lt, TwBuffer{make_rsrc(a.tw, (N2 - E) * 8u)}, L, Lf, slot);
+    constexpr int RAD = rv_R(N2, VAR);
+    stockham_passes<N2, E, DIR, XM, TWM, RAD>(v, lt, TwBuffer{make_rsrc(a.tw, (N2 - RAD) * 8u)}, L, Lf,
+                                              slot);
 
     // v[c] now holds X[N1*k2 + k1] for k2 = lt + T*c.
     if constexpr (KIND != KIND_R2C) {
@@ -127,68 +129,44 @@ __global__ void __launch_bounds__(rv_T(N2, VAR)) fft_split_kernel(RowArgs a) {
             bstore(rout, (unsigned)(N1 * lt) * 8u, (unsigned)(N1 * T * c + k1) * 8u, v[c]);
         });
     } else {
-        // Bin k = 2*k2 + k1 from Z[k] and its partner Z[NT - k] = v at
-        // k2' = (N2 - k2 - k1) mod N2 (emit as in rows_impl.h).
+        // Bin k = 2*k2 + k1 needs Z[k] and Z[NT - k], which is v at
+        // k2' = N2 - k2 - k1 (mod N2) of this workgroup.  As in rows_impl.h the
+        // upper half (k2 >= N2/2) goes through LDS once and the owner of each
+        // lower-half point emits both bins of its pair.
         const float scale = adj ? 1.0f : 0.5f;
-        auto emit = [&](int c_sp, float2 za, float2 zn, bool dc) {
-            const unsigned vo = (unsigned)(2 * lt) * 8u;
-            const unsigned so = (unsigned)(2 * c_sp + k1) * 8u;
-            if (dc) {
-                const float r0 = za.x + za.y, rn = za.x - za.y;
+        // Bin 2*(vp + sp) + k1 from za = Z[k] and zn = Z[NT - k].
+        auto emit = [&](int vp, int sp, float2 za, float2 zn) {
+            const unsigned vo = (unsigned)(2 * vp) * 8u;
+            const unsigned so = (unsigned)(2 * sp + k1) * 8u;
+            const float2 zb = cconj(zn);
+            const float2 fe = cadd(za, zb);
+            const float2 fo = cmul(bload(rrtw, vo, so), csub(za, zb));
+            bstore(rout, vo, so, cscale(cadd(fe, fo), scale));
+            if (full) bstore(rout, vo, so + (unsigned)NT * 8u, cscale(csub(fe, fo), scale));
+        };
+        float2 *Lc = SPLIT ? reinterpret_cast<float2 *>(Lf) : L;
+        static_for<E / 2, E>([&](auto C) {
+            constexpr int c = decltype(C)::value;
+            Lc[slot(lt + T * c - N2 / 2)] = v[c];
+        });
+        __syncthreads();
+        static_for<0, E / 2>([&](auto C) {
+            constexpr int c = decltype(C)::value;
+            if (c == 0 && lt == 0 && k1 == 0) {
+                // Bins 0 and NT from Z[0]; bin NT/2 (k2 = N2/2) is its own partner.
+                const float2 z0 = v[0];
+                const float r0 = z0.x + z0.y, rn = z0.x - z0.y;
                 bstore(rout, 0u, 0u, make_float2(r0, adj ? -rn : 0.f));
                 bstore(rout, 0u, (unsigned)NT * 8u, make_float2(rn, adj ? -r0 : 0.f));
+                const float2 zh = Lc[slot(0)];
+                emit(0, N2 / 2, zh, zh);
             } else {
-                const float2 zb = cconj(zn);
-                const float2 fe = cadd(za, zb);
-                const float2 fo = cmul(bload(rrtw, vo, so), csub(za, zb));
-                bstore(rout, vo, so, cscale(cadd(fe, fo), scale));
-                if (full) bstore(rout, vo, so + (unsigned)NT * 8u, cscale(csub(fe, fo), scale));
+                // Partner k2' = N2 - k2 - k1 = (T - lt - k1) + T*(E-1-c), upper half.
+                const float2 zn = Lc[slot(N2 / 2 - lt - T * c - k1)];
+                emit(lt, T * c, v[c], zn);
+                emit(T - lt - k1, T * (E - 1 - c), zn, v[c]);
             }
-        };
-        if constexpr (!SPLIT) {
-            static_for<0, E>([&](auto C) {
-                constexpr int c = decltype(C)::value;
-                L[slot(lt + T * c)] = v[c];
-            });
-            __syncthreads();
-            static_for<0, E>([&](auto C) {
-                constexpr int c = decltype(C)::value;
-                const int k2 = lt + T * c;
-                const float2 zn = L[slot((N2 - k2 - k1) & (N2 - 1))];
-                emit(T * c, v[c], zn, c == 0 && lt == 0 && k1 == 0);
-            });
-        } else {
-            // Half the points at a time through the split-exchange LDS:
-            // round A publishes k2 >= N2/2 and resolves k2 < N2/2 (whose
-            // partners are all in the upper half, bin 0 aside); round B the
-            // other way round.  With k1 = 0, k2 = N2/2 is its own partner.
-            float2 *Lc = reinterpret_cast<float2 *>(Lf);
-            static_for<E / 2, E>([&](auto C) {
-                constexpr int c = decltype(C)::value;
-                Lc[slot(lt + T * c - N2 / 2)] = v[c];
-            });
-            __syncthreads();
-            static_for<0, E / 2>([&](auto C) {
-                constexpr int c = decltype(C)::value;
-                const int k2 = lt + T * c;
-                const bool dc = c == 0 && lt == 0 && k1 == 0;
-                const float2 zn = Lc[slot(dc ? 0 : N2 - k2 - k1 - N2 / 2)];
-                emit(T * c, v[c], dc ? v[c] : zn, dc);
-            });
-            __syncthreads();
-            static_for<0, E / 2>([&](auto C) {
-                constexpr int c = decltype(C)::value;
-                Lc[slot(lt + T * c)] = v[c];
-            });
-            __syncthreads();
-            static_for<E / 2, E>([&](auto C) {
-                constexpr int c = decltype(C)::value;
-                const int k2 = lt + T * c;
-                const bool self = c == E / 2 && lt == 0 && k1 == 0;
-                const float2 zn = Lc[slot(self ? 0 : N2 - k2 - k1)];
-                emit(T * c, v[c], self ? v[c] : zn, false);
-            });
-        }
+        });
     }
 }
 

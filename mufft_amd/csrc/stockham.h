@@ -79,18 +79,21 @@ MUFFT_HD constexpr int np_for(int N, int E) {
 }
 
 // v[c] holds point lt + T*c on entry (T = N/E) and holds output point lt + T*c
-// on exit.  `twl` reads the twiddle table of the sequence (E, ..., E, r_last).
-// `Slot(o)` maps a point index to its LDS slot; `L` is the complex LDS view,
-// `Lf` the float view used by the split exchange.
-template <int N, int E, int DIR, int XM, int TWM, typename TwLoad, typename Slot>
+// on exit.  Every pass but the last has radix R (R <= E, default E); each
+// thread runs J = E/r butterflies of a pass, butterfly j being b = lt + T*j
+// over its points v[j + m*J].  `twl` reads the twiddle table of the sequence
+// (R, ..., R, r_last).  `Slot(o)` maps a point index to its LDS slot; `L` is
+// the complex LDS view, `Lf` the float view used by the split exchange.
+template <int N, int E, int DIR, int XM, int TWM, int R = E, typename TwLoad, typename Slot>
 __device__ __forceinline__ void stockham_passes(float2 (&v)[E], int lt, TwLoad twl,
                                                 float2 *L, float *Lf, Slot slot) {
     constexpr int T = N / E;
-    constexpr int NP = np_for(N, E);
+    constexpr int NP = np_for(N, R);
+    static_assert(R <= E && E % R == 0, "pass radix divides the points per thread");
     static_for<0, NP>([&](auto S) {
         constexpr int s = decltype(S)::value;
-        constexpr int r = seq_radix(N, E, NP, s);
-        constexpr int p = seq_p(E, s);
+        constexpr int r = seq_radix(N, R, NP, s);
+        constexpr int p = seq_p(R, s);
         constexpr int J = E / r;   // butterflies per thread in this pass
         static_for<0, J>([&](auto JJ) {
             constexpr int j = decltype(JJ)::value;
@@ -98,21 +101,28 @@ __device__ __forceinline__ void stockham_passes(float2 (&v)[E], int lt, TwLoad t
             static_for<0, r>([&](auto M) { u[decltype(M)::value] = v[j + decltype(M)::value * J]; });
             if constexpr (s > 0) {
                 const int k = (lt + T * j) & (p - 1);
-                apply_twiddles<r, p, TWM>(u, k, tw_offset(E, p), twl);
+                apply_twiddles<r, p, TWM>(u, k, tw_offset(R, p), twl);
             }
             dft<r, DIR>(u);
             static_for<0, r>([&](auto Q) { v[j + decltype(Q)::value * J] = u[decltype(Q)::value]; });
         });
         if constexpr (s < NP - 1) {
-            // Not the last pass: r == E and J == 1.  Output q goes to point
-            // (lt - k)*E + k + q*p; the next pass reads points lt + T*c.
-            const int k = lt & (p - 1);
-            const int base = (lt - k) * E + k;
-            if constexpr (XM != XM_SPLIT) {
-                static_for<0, E>([&](auto Q) {
-                    constexpr int q = decltype(Q)::value;
-                    L[slot(base + q * p)] = v[q];
+            // Output q of butterfly b goes to point (b - k)*r + k + q*p with
+            // k = b mod p; the next pass reads points lt + T*c.
+            auto publish = [&](auto get, auto put) {
+                static_for<0, J>([&](auto JJ) {
+                    constexpr int j = decltype(JJ)::value;
+                    const int b = lt + T * j;
+                    const int k = b & (p - 1);
+                    const int base = (b - k) * r + k;
+                    static_for<0, r>([&](auto Q) {
+                        constexpr int q = decltype(Q)::value;
+                        put(slot(base + q * p), get(v[j + q * J]));
+                    });
                 });
+            };
+            if constexpr (XM != XM_SPLIT) {
+                publish([](float2 z) { return z; }, [&](int o, float2 z) { L[o] = z; });
                 __syncthreads();
                 static_for<0, E>([&](auto C) {
                     constexpr int c = decltype(C)::value;
@@ -120,20 +130,14 @@ __device__ __forceinline__ void stockham_passes(float2 (&v)[E], int lt, TwLoad t
                 });
                 __syncthreads();
             } else {
-                static_for<0, E>([&](auto Q) {
-                    constexpr int q = decltype(Q)::value;
-                    Lf[slot(base + q * p)] = v[q].x;
-                });
+                publish([](float2 z) { return z.x; }, [&](int o, float x) { Lf[o] = x; });
                 __syncthreads();
                 static_for<0, E>([&](auto C) {
                     constexpr int c = decltype(C)::value;
                     v[c].x = Lf[slot(lt + T * c)];
                 });
                 __syncthreads();
-                static_for<0, E>([&](auto Q) {
-                    constexpr int q = decltype(Q)::value;
-                    Lf[slot(base + q * p)] = v[q].y;
-                });
+                publish([](float2 z) { return z.y; }, [&](int o, float y) { Lf[o] = y; });
                 __syncthreads();
                 static_for<0, E>([&](auto C) {
                     constexpr int c = decltype(C)::value;
