@@ -192,6 +192,42 @@ def conv_batch_bench(dev, stream, reps: int = 10):
     }
 
 
+def host_paths_bench():
+    """PCIe-inclusive rates of the host-memory paths (not part of `value`):
+    one drop-in call on pinned buffers (used in place) and on pageable buffers
+    (staged), and the c2c N=4096 x 16384 batch streamed from host to host."""
+    import torch
+
+    import mufft_amd as M
+    out = {}
+    n = 1024
+    p = M.Plan1D("c2c", n, M.FORWARD)
+    for mem in ("pinned", "pageable"):
+        x = torch.randn(n, dtype=torch.complex64, pin_memory=(mem == "pinned"))
+        y = torch.empty(n, dtype=torch.complex64, pin_memory=(mem == "pinned"))
+        for _ in range(10):
+            p.execute(y, x)
+        t0 = time.perf_counter()
+        for _ in range(200):
+            p.execute(y, x)
+        out[f"dropin_c2c_1024_{mem}_us"] = round((time.perf_counter() - t0) / 200 * 1e6, 1)
+    q = M.Plan1D("c2c", C2C_N, M.FORWARD)
+    for mem in ("pinned", "pageable"):
+        x = torch.empty((C2C_B, C2C_N), dtype=torch.complex64, pin_memory=(mem == "pinned"))
+        x.real.uniform_(-1, 1)
+        x.imag.uniform_(-1, 1)
+        y = torch.empty_like(x, pin_memory=(mem == "pinned"))
+        q.execute_batched(y, x, C2C_B)
+        t0 = time.perf_counter()
+        for _ in range(3):
+            q.execute_batched(y, x, C2C_B)
+        dt = (time.perf_counter() - t0) / 3
+        out[f"batched_host_c2c_4096x16384_{mem}_ms"] = round(dt * 1e3, 2)
+        out[f"batched_host_c2c_4096x16384_{mem}_GFLOPs"] = round(5.0 * C2C_N * math.log2(C2C_N) * C2C_B / dt / 1e9, 1)
+        del x, y
+    return out
+
+
 def diff_conv_bench(dev, stream, steps: int = 10, warmup: int = 3):
     """Differentiable convolution training step: 64 stereo signals, N = 65536,
     learnable real filter of 32768 taps (N(0, (1e-2)^2)), MSE to the outputs of a
@@ -246,8 +282,8 @@ def diff_conv_bench(dev, stream, steps: int = 10, warmup: int = 3):
         "ms_per_step": round(ms, 4),
         "signals_per_s": round(B / (ms * 1e-3), 1),
         "filter_grad_rel_rms_vs_fp64": rel,
-        "loss_first": float(loss0),
-        "loss_last": float(last),
+        "loss_first": float(loss0.detach()),
+        "loss_last": float(last.detach()),
     }
 
 
@@ -348,6 +384,7 @@ def main():
     torch.cuda.empty_cache()
     conv = conv_batch_bench(dev, stream)
     dconv = diff_conv_bench(dev, stream)
+    host = host_paths_bench()
 
     out = {
         "metric": METRIC,
@@ -390,6 +427,7 @@ def main():
         "conv_batch_speedup": conv["speedup"],
         "conv_batch": conv,
         "diff_conv": dconv,
+        "host_paths": host,
         "input_gen_s": round(gen_s, 2),
     }
     print(json.dumps(out))

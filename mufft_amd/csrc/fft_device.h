@@ -116,6 +116,18 @@ __device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, unsigned voff, 
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, (int)voff, (int)soff, 0);
 }
 
+// Streaming variants (cache policy "nt", aux bit 1): for data that one
+// kernel reads or writes exactly once, so it does not displace reused lines
+// (twiddle tables, operands read twice) from the caches.
+__device__ __forceinline__ float2 bload_nt(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 2));
+}
+
+__device__ __forceinline__ void bstore_nt(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff,
+                                          float2 v) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, (int)voff, (int)soff, 2);
+}
+
 // Padded LDS slot of point o when every E points get one spare slot.
 template <int LOGE>
 __device__ __forceinline__ int pad_slot(int o) { return o + (o >> LOGE); }

@@ -54,8 +54,11 @@ __device__ __forceinline__ void row_transform(const float2 *X, const float2 *Y, 
         rout = make_rsrc(Xo, active ? nout * 8u : 0u);
         rrtw = make_rsrc(rtw, KIND == KIND_C2C ? 0u : N * 8u);
     }
+    // c2c / r2c inputs are read once (streaming); c2r reads each bin twice
+    // (as X[k] and as X[N - k]), so its input stays cacheable.
     auto ld_in = [&](int vp, int sp) -> float2 {
-        if constexpr (BUF) return bload(rin, (unsigned)vp * 8u, (unsigned)sp * 8u);
+        if constexpr (BUF && KIND != KIND_C2R) return bload_nt(rin, (unsigned)vp * 8u, (unsigned)sp * 8u);
+        else if constexpr (BUF) return bload(rin, (unsigned)vp * 8u, (unsigned)sp * 8u);
         else return X[vp + sp];
     };
     auto ld_in2 = [&](int vp, int sp) -> float2 {
@@ -67,7 +70,7 @@ __device__ __forceinline__ void row_transform(const float2 *X, const float2 *Y, 
         else return rtw[vp + sp];
     };
     auto st_out = [&](int vp, int sp, float2 val) {
-        if constexpr (BUF) bstore(rout, (unsigned)vp * 8u, (unsigned)sp * 8u, val);
+        if constexpr (BUF) bstore_nt(rout, (unsigned)vp * 8u, (unsigned)sp * 8u, val);
         else Xo[vp + sp] = val;
     };
 

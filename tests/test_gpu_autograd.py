@@ -97,5 +97,5 @@ def test_full_size_stereo_gradient_and_training(gpu):
         l = torch.mean(torch.abs(op(tx, th) - target) ** 2)
         l.backward()
         opt.step()
-        losses.append(float(l))
+        losses.append(float(l.detach()))
     assert losses[-1] < losses[0]
