@@ -136,7 +136,7 @@ mufft_hip_conv_batch *mufft_hip_create_conv_batch(const mufft_hip_conv_job *jobs
 int mufft_hip_execute_conv_batch(mufft_hip_conv_batch *batch, void *stream);
 /* Same kernels without the graph. */
 int mufft_hip_execute_conv_batch_direct(mufft_hip_conv_batch *batch, void *stream);
-/* 0 = last execution complete, 1 = a dependency wait timed out. */
+/* 0 = the last execution ran every task, 1 = it did not (synchronises the device). */
 int mufft_hip_conv_batch_status(mufft_hip_conv_batch *batch);
 int mufft_hip_conv_batch_info(const mufft_hip_conv_batch *batch, int *ntasks, int *grid);
 void mufft_hip_free_conv_batch(mufft_hip_conv_batch *batch);

@@ -1779,8 +1779,9 @@ int mufft_hip_tune_load(const char *path) {
 }  // extern "C"
 
 // ===========================================================================
-// Uneven convolution batch: one persistent launch for many convolutions of
-// mixed sizes and methods, replayed from a captured hipGraph.
+// Uneven convolution batch: two persistent launches (forward tasks, then
+// inverse tasks) for many convolutions of mixed sizes and methods, replayed
+// from a captured hipGraph.
 // ===========================================================================
 struct mufft_hip_conv_batch {
     int device = 0;
@@ -1789,7 +1790,7 @@ struct mufft_hip_conv_batch {
     std::vector<CBJob> jobs;
     CBArgs args{};
     int grid = 0;
-    DevBuf d_tasks, d_jobs, d_counters, d_flags, d_spectra;
+    DevBuf d_tasks, d_jobs, d_flags, d_spectra;
     std::vector<float2 *> tables;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -1806,7 +1807,6 @@ void destroy_batch(mufft_hip_conv_batch *b) {
         if (t) (void)hipFree(t);
     b->d_tasks.release();
     b->d_jobs.release();
-    b->d_counters.release();
     b->d_flags.release();
     b->d_spectra.release();
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -1980,8 +1980,7 @@ mufft_hip_conv_batch *mufft_hip_create_conv_batch(const mufft_hip_conv_job *jobs
     b->grid = std::max(1, std::min<int>(prop.multiProcessorCount, (int)b->tasks.size()));
     if (b->d_tasks.ensure(b->tasks.size() * sizeof(CBTask)) ||
         b->d_jobs.ensure(b->jobs.size() * sizeof(CBJob)) ||
-        b->d_counters.ensure(std::max<size_t>(count, 1) * sizeof(int)) ||
-        b->d_flags.ensure(2 * sizeof(int)) ||
+        b->d_flags.ensure(4 * sizeof(int)) ||
         hipMemcpyAsync(b->d_tasks.p, b->tasks.data(), b->tasks.size() * sizeof(CBTask),
                        hipMemcpyHostToDevice, b->stream) != hipSuccess ||
         hipMemcpyAsync(b->d_jobs.p, b->jobs.data(), b->jobs.size() * sizeof(CBJob),
@@ -1993,14 +1992,13 @@ mufft_hip_conv_batch *mufft_hip_create_conv_batch(const mufft_hip_conv_job *jobs
     }
     CBArgs &a = b->args;
     a.tasks = static_cast<const CBTask *>(b->d_tasks.p);
+    a.nfwd = (int)fwd.size();
     a.ntasks = (int)b->tasks.size();
     a.jobs = static_cast<const CBJob *>(b->d_jobs.p);
     a.njobs = (int)count;
-    a.counters = static_cast<int *>(b->d_counters.p);
     a.head = static_cast<int *>(b->d_flags.p);
-    a.error = a.head + 1;
-    a.spin_limit = 1LL << 22;  // ~0.1-0.3 s per task: a bound, never reached in normal runs
-    // Capture reset + persistent kernel once; executions replay the graph.
+    a.done = a.head + 2;
+    // Capture reset + the two persistent launches once; executions replay the graph.
     if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess ||
         launch_conv_batch(a, b->grid, b->stream) != hipSuccess ||
         hipStreamEndCapture(b->stream, &b->graph) != hipSuccess ||
@@ -2035,14 +2033,14 @@ int mufft_hip_execute_conv_batch_direct(mufft_hip_conv_batch *b, void *stream) {
     return 0;
 }
 
-// 0 when the last execution finished every task, 1 when a dependency wait
-// timed out (its outputs are then incomplete).  Synchronises the batch stream.
+// 0 when the last execution ran every task, 1 otherwise.  Synchronises the
+// device (the batch may have run on a caller's stream).
 int mufft_hip_conv_batch_status(mufft_hip_conv_batch *b) {
     if (!b) return -1;
-    int err = 0;
-    HIP_TRY(hipStreamSynchronize(b->stream));
-    HIP_TRY(hipMemcpy(&err, b->args.error, sizeof(int), hipMemcpyDeviceToHost));
-    return err;
+    int done = 0;
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(&done, b->args.done, sizeof(int), hipMemcpyDeviceToHost));
+    return done == b->args.ntasks ? 0 : 1;
 }
 
 int mufft_hip_conv_batch_info(const mufft_hip_conv_batch *b, int *ntasks, int *grid) {

@@ -34,15 +34,15 @@ struct CBTask {
     const float2 *rtw;     // real-transform resolve twiddles (r2c / c2r)
 };
 
+// Tasks [0, nfwd) are the forward transforms, [nfwd, ntasks) the inverse
+// transforms; each phase is one persistent launch with its own queue head.
 struct CBArgs {
     const CBTask *tasks;
-    int ntasks;
+    int nfwd, ntasks;
     const CBJob *jobs;
     int njobs;
-    int *counters;         // per job: forward tasks completed
-    int *head;             // queue head
-    int *error;            // set when a dependency wait exceeds spin_limit
-    long long spin_limit;
+    int *head;             // queue heads of the two phases
+    int *done;             // tasks completed (both phases)
 };
 
 hipError_t launch_conv_batch(const CBArgs &a, int grid, hipStream_t s);

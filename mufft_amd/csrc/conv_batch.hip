@@ -3,17 +3,17 @@
 // The reference runs one convolution as three calls (fft.c:799-809): forward
 // transform of each input block, then multiply + inverse transform.  Here a
 // whole batch of such jobs, of any mix of sizes and of both methods (mono:
-// r2c / r2c / c2r, stereo: c2c / full r2c / c2c, fft.c:583-600), runs in one
-// launch.  Each job is split into three tasks: forward(first), forward(second)
-// and inverse (with the spectrum multiply fused into its prologue).  Tasks of
-// equal shape are grouped up to one workgroup's capacity; the host orders all
-// forward tasks before all inverse tasks, each part longest-first (LPT).
-// Workgroups pull tasks from a device queue (an atomic counter).  A forward
-// task publishes its spectra (release fence at agent scope) and increments
-// its jobs' counters; an inverse task waits until both counters of its jobs
-// reach 2 (acquire).  Forward tasks never wait and every task is pulled by a
-// running workgroup, so the waits always end; a spin limit nevertheless bounds
-// them and raises an error flag instead of hanging.
+// r2c / r2c / c2r, stereo: c2c / full r2c / c2c, fft.c:583-600), runs in two
+// persistent launches: one over all forward tasks (both inputs of every job),
+// one over all inverse tasks (with the spectrum multiply fused into their
+// prologue).  Tasks of equal shape are grouped up to one workgroup's
+// capacity and each phase is ordered longest-first (LPT); workgroups pull
+// tasks from a device queue (an atomic counter).  The launch boundary orders
+// the phases.  A device-wide wait inside one launch would need agent-scope
+// release/acquire fences, which on gfx950 write back or invalidate a whole
+// XCD's L2 (buffer_wbl2 / buffer_inv) at every task and slowed the batch
+// five-fold; the queue atomics here are relaxed and no workgroup ever waits
+// for another.
 #include "conv_batch.h"
 #include "rows_impl.h"
 
@@ -76,67 +76,49 @@ __device__ __forceinline__ void cb_dispatch_sizes(const CBTask &t, const CBJob *
     });
 }
 
-__global__ void __launch_bounds__(CB_WG) conv_batch_kernel(CBArgs a) {
+__global__ void __launch_bounds__(CB_WG) conv_batch_kernel(CBArgs a, int phase) {
     __shared__ float2 smem[CB_LDS_BYTES / 8];
     __shared__ int s_task;
     const int tid = threadIdx.x;
+    const int begin = phase == 0 ? 0 : a.nfwd;
+    const int end = phase == 0 ? a.nfwd : a.ntasks;
     for (;;) {
-        if (tid == 0) s_task = atomicAdd(a.head, 1);
+        if (tid == 0) s_task = begin + atomicAdd(a.head + phase, 1);
         __syncthreads();
         const int ti = s_task;
         __syncthreads();
-        if (ti >= a.ntasks) break;
+        if (ti >= end) break;
         const CBTask t = a.tasks[ti];
-        if (t.type == CB_INV) {
-            if (tid == 0) {
-                long long spins = 0;  // one budget for the whole task
-                for (int j = 0; j < t.njobs && spins <= a.spin_limit; ++j) {
-                    int *c = a.counters + t.job0 + j;
-                    while (__hip_atomic_load(c, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < 2) {
-                        __builtin_amdgcn_s_sleep(2);
-                        if (++spins > a.spin_limit) {
-                            atomicExch(a.error, 1);
-                            break;
-                        }
-                    }
-                }
-            }
-            __syncthreads();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        }
         if (t.kind == KIND_R2C) cb_dispatch_sizes<KIND_R2C, -1, CB_MIN_LOG2, 15>(t, a.jobs, tid, smem);
         else if (t.kind == KIND_C2R) cb_dispatch_sizes<KIND_C2R, 1, CB_MIN_LOG2, 15>(t, a.jobs, tid, smem);
         else if (t.dir < 0) cb_dispatch_sizes<KIND_C2C, -1, CB_MIN_LOG2, 15>(t, a.jobs, tid, smem);
         else cb_dispatch_sizes<KIND_C2C, 1, CB_MIN_LOG2, 15>(t, a.jobs, tid, smem);
-        if (t.type != CB_INV) {
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            __syncthreads();
-            if (tid < t.njobs) {
-                __hip_atomic_fetch_add(a.counters + t.job0 + tid, 1, __ATOMIC_RELEASE,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-            }
-        }
+        if (tid == 0) atomicAdd(a.done, 1);
     }
 }
 
-// Clears the queue head, the error flag and the per-job counters (a kernel,
-// so the reset can live inside a captured graph).
-__global__ void __launch_bounds__(256) conv_batch_reset_kernel(CBArgs a) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < a.njobs) a.counters[i] = 0;
-    if (i == 0) {
-        *a.head = 0;
-        *a.error = 0;
-    }
+// Clears the queue heads and the completion count (a kernel, so the reset can
+// live inside a captured graph).
+__global__ void __launch_bounds__(64) conv_batch_reset_kernel(CBArgs a) {
+    if (threadIdx.x < 2) a.head[threadIdx.x] = 0;
+    if (threadIdx.x == 0) *a.done = 0;
 }
 
 hipError_t launch_conv_batch(const CBArgs &a, int grid, hipStream_t s) {
-    const int nreset = (a.njobs + 255) / 256;
-    hipLaunchKernelGGL(conv_batch_reset_kernel, dim3(nreset > 0 ? nreset : 1), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(conv_batch_reset_kernel, dim3(1), dim3(64), 0, s, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(conv_batch_kernel, dim3(grid), dim3(CB_WG), 0, s, a);
-    return hipGetLastError();
+    const int g0 = a.nfwd < grid ? a.nfwd : grid;
+    const int g1 = a.ntasks - a.nfwd < grid ? a.ntasks - a.nfwd : grid;
+    if (g0 > 0) {
+        hipLaunchKernelGGL(conv_batch_kernel, dim3(g0), dim3(CB_WG), 0, s, a, 0);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    if (g1 > 0) {
+        hipLaunchKernelGGL(conv_batch_kernel, dim3(g1), dim3(CB_WG), 0, s, a, 1);
+        e = hipGetLastError();
+    }
+    return e;
 }
 
 }  // namespace mufft
