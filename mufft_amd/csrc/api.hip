@@ -349,7 +349,7 @@ int tune_kind(int kind, int dir) {
 int chosen_variant(int logn, int tk) {
     std::lock_guard<std::mutex> g(g_var_lock);
     const int v = g_variant[logn][tk];
-    return v ? v : rv_default(1 << logn);
+    return v ? v : rv_default(1 << logn, tk >= 2);
 }
 
 // A complex transform of length n along one axis: fused row kernel, fused
@@ -472,7 +472,7 @@ int run_large(const Engine &e, const float2 *src, long long sd, float2 *dst, lon
     f.in_bs = sd;
     f.out_bs = e.n;
     f.n2 = n2;
-    f.ntiles = n2 / col_TX(n1);
+    f.ntiles = n2 / fs_TX(n1);
     f.batch = batch;
     f.log2s = e.log_s;
     f.flags = zp ? RF_ZERO_PAD : 0u;
@@ -1113,7 +1113,8 @@ static int exec_2d(mufft_plan_2d *p, void *out_, const void *in_, unsigned howma
             if (p->s2.ensure(sizeof(float2) * (size_t)img * hm)) return -1;
             rows_dst = p->s2.f2();
         }
-        const unsigned f = zp | (p->full ? RF_FULL : 0u);
+        // The column pass reads the rows' output next: keep it cacheable.
+        const unsigned f = zp | (p->full ? RF_FULL : 0u) | RF_KEEP_OUT;
         if (run_rows(p->ex, p->kind, p->rtw, in, Nx, nullptr, 0, rows_dst, rs, (int)(Ny * hm), f,
                      1.0f, p->s0, p->s1, st))
             return -1;
@@ -1600,7 +1601,7 @@ int mufft_hip_default_variant(unsigned N, int kind) {
         set_err("no fused kernel for this size");
         return -1;
     }
-    return rv_default(1 << l);
+    return rv_default(1 << l, tk >= 2);
 }
 
 int mufft_hip_get_variant(unsigned N, int kind) {
@@ -1946,6 +1947,7 @@ mufft_hip_conv_batch *mufft_hip_create_conv_batch(const mufft_hip_conv_job *jobs
                 t.job0 = (int)q;
                 t.njobs = (int)(q2 - q);
                 t.flags = sh.flags;
+                if (sh.type != CB_INV) t.flags |= RF_KEEP_OUT;   // spectra are read by the inverse phase
                 if (sh.type == CB_FWD_FIRST && (m0 & MUFFT_CONV_METHOD_FLAG_ZERO_PAD_UPPER_HALF_FIRST))
                     t.flags |= RF_ZERO_PAD;
                 if (sh.type == CB_FWD_SECOND && (m0 & MUFFT_CONV_METHOD_FLAG_ZERO_PAD_UPPER_HALF_SECOND))
@@ -2139,9 +2141,11 @@ int mufft_hip_execute_ola(mufft_hip_ola *o, float *output, const float *input, u
         o->Y.ensure(sizeof(float) * (size_t)n * nblocks))
         return -1;
     // 1. spectra of every hop (zero padded to N); block b starts at input + b*h.
-    if (exec_1d(o->fwd, o->X.p, input, nullptr, nblocks, h / 2, 0, h + 1, 0u, 1.0f, st)) return -1;
+    if (exec_1d(o->fwd, o->X.p, input, nullptr, nblocks, h / 2, 0, h + 1, RF_KEEP_OUT, 1.0f, st))
+        return -1;
     // 2. filtered blocks: c2r of X * H / N (H broadcast to every block).
-    if (exec_1d(o->inv, o->Y.p, o->X.p, o->H.p, nblocks, h + 1, 0, h, RF_MUL, 1.0f / (float)n, st))
+    if (exec_1d(o->inv, o->Y.p, o->X.p, o->H.p, nblocks, h + 1, 0, h, RF_MUL | RF_KEEP_OUT,
+                1.0f / (float)n, st))
         return -1;
     // 3. overlap-add with the tail of the previous call.
     float *tail_in = static_cast<float *>(o->tail[o->cur].p);

@@ -20,7 +20,8 @@ __global__ void __launch_bounds__(col_WG(L_)) fft_cols_kernel(ColArgs a) {
     constexpr int T = col_T(L_);
     constexpr int TX = col_TX(L_);
     constexpr int ROWS = col_lds_rows(L_);
-    __shared__ float2 smem[ROWS * TX];
+    constexpr int XM = col_xm(L_);
+    __shared__ float2 smem[XM == XM_SPLIT ? (ROWS * TX + 1) / 2 : ROWS * TX];
 
     const int tid = threadIdx.x;
     const int x = tid & (TX - 1);
@@ -39,7 +40,8 @@ __global__ void __launch_bounds__(col_WG(L_)) fft_cols_kernel(ColArgs a) {
         v[c] = active ? src[(long long)(t + T * c) * a.in_rs] : make_float2(0.f, 0.f);
     });
 
-    stockham_passes<L_, E, DIR, XM_PAD, TW_FULL>(v, t, TwGlobal{a.tw}, smem, nullptr, slot);
+    stockham_passes<L_, E, DIR, XM, col_twm(L_)>(v, t, TwGlobal{a.tw}, smem, reinterpret_cast<float *>(smem),
+                                                 slot);
 
     if (active) {
         float2 *dst = a.out + (long long)img * a.out_bs + col;
@@ -54,12 +56,12 @@ __global__ void __launch_bounds__(col_WG(L_)) fft_cols_kernel(ColArgs a) {
 // w_N^(dir*n2*k1) twiddle, stored transposed through LDS so that each column
 // of the tile leaves as one contiguous run of N1 points.
 template <int L_, int DIR>
-__global__ void __launch_bounds__(col_WG(L_)) fft_four_step_cols_kernel(FourStepArgs a) {
+__global__ void __launch_bounds__(fs_WG(L_)) fft_four_step_cols_kernel(FourStepArgs a) {
     constexpr int E = col_E(L_);
     constexpr int LOGE = ilog2(E);
     constexpr int T = col_T(L_);
-    constexpr int TX = col_TX(L_);
-    constexpr int WG = col_WG(L_);
+    constexpr int TX = fs_TX(L_);
+    constexpr int WG = fs_WG(L_);
     constexpr int ROWS = col_lds_rows(L_);
     constexpr int LP = L_ + 1;   // row pitch of the transposed tile
     static_assert(LP <= ROWS, "transposed tile fits the exchange LDS");
@@ -120,13 +122,13 @@ hipError_t launch_four_step_cols(int logN1, int direction, const FourStepArgs &a
     const void *k = direction < 0 ? four_step_kernel_ptr<-1>(logN1) : four_step_kernel_ptr<1>(logN1);
     if (!k) return hipErrorInvalidValue;
     const int L = 1 << logN1;
-    if (a.n2 % col_TX(L) != 0 || a.ntiles != a.n2 / col_TX(L)) return hipErrorInvalidValue;
+    if (a.n2 % fs_TX(L) != 0 || a.ntiles != a.n2 / fs_TX(L)) return hipErrorInvalidValue;
     const long long nblocks = (long long)a.ntiles * a.batch;
     if (nblocks <= 0) return hipSuccess;
     if (nblocks > 0x7fffffffLL) return hipErrorInvalidValue;
     FourStepArgs args = a;
     void *params[] = {&args};
-    return hipLaunchKernel(k, dim3((unsigned)nblocks), dim3(col_WG(L)), params, 0, s);
+    return hipLaunchKernel(k, dim3((unsigned)nblocks), dim3(fs_WG(L)), params, 0, s);
 }
 
 template <int DIR>

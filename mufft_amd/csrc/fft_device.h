@@ -136,9 +136,11 @@ __device__ __forceinline__ int pad_slot(int o) { return o + (o >> LOGE); }
 // dispatches workgroups round-robin over its 8 XCDs, each with its own L2):
 // neighbouring column tiles share cache lines, so keeping them on one L2
 // avoids fetching the same line into several L2s.
+// The blocks beyond the last multiple of 8 keep their own index.
 __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) {
-    if ((nb & 7u) != 0u) return b;
-    return (b & 7u) * (nb >> 3) + (b >> 3);
+    const unsigned full = nb & ~7u;
+    if (b >= full) return b;
+    return (b & 7u) * (full >> 3) + (b >> 3);
 }
 
 }  // namespace mufft

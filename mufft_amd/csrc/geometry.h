@@ -26,6 +26,9 @@ enum RowFlag : unsigned {
     // Adjoint weightings (gradient convention dL/dRe + i dL/dIm):
     RF_ADJ_R2C = 1u << 4,    // c2r prologue: interior bins x 1/2, Im of DC and Nyquist dropped
     RF_ADJ_C2R = 1u << 5,    // r2c epilogue: interior bins x 2, DC/Nyquist cross terms
+    // The next kernel reads the output: store it cacheable (default: streaming
+    // stores, for outputs that leave the device's caches anyway).
+    RF_KEEP_OUT = 1u << 6,
 };
 
 MUFFT_HD constexpr int ilog2(long long n) { return n <= 1 ? 0 : 1 + ilog2(n >> 1); }
@@ -100,16 +103,17 @@ MUFFT_HD constexpr int rv_lds_per(int N, int var) {
 MUFFT_HD constexpr long long rv_lds_bytes(int N, int var) {
     return (long long)rv_B(N, var) * rv_lds_per(N, var) * (rv_xm(var) == 1 ? 4 : 8);
 }
-// Static choice per size, from a sweep of all registered variants on MI355X
-// (c2c, batches of 2^27 points): split-product twiddles pay off from 1024
-// points on; 16384 prefers 32 points per thread; 32768 must split the LDS
-// exchange to fit one workgroup.
-MUFFT_HD constexpr int rv_default(int N) {
+// Static choice per size and transform kind (c2c, or r2c/c2r whose fused
+// transform has N = half the real length), from sweeps of all registered
+// variants on MI355X at 2^27-2^28 points per launch: split-product twiddles
+// pay off from 1024 points on; c2c prefers 32 points per thread at 1024 and
+// 16384 while the real transforms, whose resolve adds registers, prefer 16;
+// 32768 needs the split LDS exchange to fit one workgroup; 65536 runs on two
+// workgroups per transform.
+MUFFT_HD constexpr int rv_default(int N, bool real = false) {
     return N <= 512 ? rv_make(16, 0, 0)
-         : N == 1024 ? rv_make(32, 0, 1)
-         : N <= 8192 ? rv_make(16, 0, 1)
-         : N == 16384 ? rv_make(32, 0, 1)
-         : N == 32768 ? rv_make_split(1, rv_make(32, 0, 1))
+         : N <= 16384 ? ((N == 1024 || N == 16384) && !real ? rv_make(32, 0, 1) : rv_make(16, 0, 1))
+         : N == 32768 ? rv_make(32, 1, 1)
          : rv_make_split(1, rv_make(32, 1, 1));
 }
 
@@ -126,9 +130,17 @@ MUFFT_HD constexpr int seq_p(int E, int s) { return 1 << (ilog2(E) * s); }
 // ---------------------------------------------------------------------------
 MUFFT_HD constexpr int col_E(int L) { return row_E(L); }
 MUFFT_HD constexpr int col_T(int L) { return L / col_E(L); }
-MUFFT_HD constexpr int col_TX(int L) {
+// Tile width of the four-step column kernel (whose transposed store needs the
+// full complex tile in LDS).
+MUFFT_HD constexpr int fs_TX(int L) {
     return col_T(L) <= 32 ? 256 / col_T(L) : (col_T(L) <= 128 ? 8 : (L >= 8192 ? 2 : 4));
 }
+MUFFT_HD constexpr int fs_WG(int L) { return col_T(L) * fs_TX(L); }
+// The plain column kernel: at L = 1024 a 16-column tile (one 128-byte line
+// per row) with the split LDS exchange and split twiddles measured 15% faster.
+MUFFT_HD constexpr int col_TX(int L) { return L == 1024 ? 16 : fs_TX(L); }
+MUFFT_HD constexpr int col_xm(int L) { return L == 1024 ? 1 : 0; }    // XMode
+MUFFT_HD constexpr int col_twm(int L) { return L == 1024 ? 1 : 0; }   // TwMode
 MUFFT_HD constexpr int col_WG(int L) { return col_T(L) * col_TX(L); }
 MUFFT_HD constexpr int col_NP(int L) { return row_NP(L); }
 MUFFT_HD constexpr int col_lds_rows(int L) { return L + L / col_E(L); }

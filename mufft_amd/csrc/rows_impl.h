@@ -69,9 +69,14 @@ __device__ __forceinline__ void row_transform(const float2 *X, const float2 *Y, 
         if constexpr (BUF) return bload(rrtw, (unsigned)vp * 8u, (unsigned)sp * 8u);
         else return rtw[vp + sp];
     };
+    const bool keep = (flags & RF_KEEP_OUT) != 0;
     auto st_out = [&](int vp, int sp, float2 val) {
-        if constexpr (BUF) bstore_nt(rout, (unsigned)vp * 8u, (unsigned)sp * 8u, val);
-        else Xo[vp + sp] = val;
+        if constexpr (BUF) {
+            if (keep) bstore(rout, (unsigned)vp * 8u, (unsigned)sp * 8u, val);
+            else bstore_nt(rout, (unsigned)vp * 8u, (unsigned)sp * 8u, val);
+        } else {
+            Xo[vp + sp] = val;
+        }
     };
 
     float2 v[E];
