@@ -150,6 +150,16 @@ def conv_batch_bench(dev, stream, reps: int = 10):
     torch.cuda.synchronize()
     t_batch = e0.elapsed_time(e1) / reps * 1e-3
     assert batch.status() == 0
+    # the same kernels launched directly (reset + two persistent phases, no graph)
+    e2 = torch.cuda.Event(enable_timing=True)
+    e3 = torch.cuda.Event(enable_timing=True)
+    e2.record(stream)
+    for _ in range(reps):
+        batch.execute(stream=stream, graph=False)
+    e3.record(stream)
+    torch.cuda.synchronize()
+    t_direct = e2.elapsed_time(e3) / reps * 1e-3
+    assert batch.status() == 0
     errs = []
     for k in (0, 1, 2, 511, 512, 639):
         N, method, _, _, out = jobs[k]
@@ -184,6 +194,7 @@ def conv_batch_bench(dev, stream, reps: int = 10):
         "tasks": ntasks,
         "grid": grid,
         "persistent_graph_ms": round(t_batch * 1e3, 4),
+        "persistent_direct_ms": round(t_direct * 1e3, 4),
         "per_plan_dropin_ms": round(t_dropin * 1e3, 3),
         "speedup": round(t_dropin / t_batch, 2),
         "alg_bytes": total_bytes,

@@ -189,9 +189,8 @@ struct StagedOperand {
 // Runs run(dev_out, dev_in, count) over `howmany` units in chunks; returns
 // when the last chunk's output is in host memory.
 template <typename Run>
-int run_staged(HostPipe &pp, hipStream_t comp, StagedOperand o, StagedOperand i, bool preload_out,
-               unsigned howmany, Run run) {
-    if (pp.init()) return -1;
+int run_staged_chunks(HostPipe &pp, hipStream_t comp, StagedOperand o, StagedOperand i,
+                      bool preload_out, unsigned howmany, Run run) {
     const size_t big = std::max(o.stride, i.stride);
     const unsigned per = (unsigned)std::max<size_t>(1, HostPipe::kChunkBytes / std::max<size_t>(big, 1));
     const unsigned nchunks = (howmany + per - 1) / per;
@@ -241,6 +240,21 @@ int run_staged(HostPipe &pp, hipStream_t comp, StagedOperand o, StagedOperand i,
     HIP_TRY(hipStreamSynchronize(pp.up));
     HIP_TRY(hipStreamSynchronize(comp));
     return 0;
+}
+
+// As run_staged_chunks; after a failure the three streams are drained, so no
+// copy into the staging slots is still in flight when the call returns.
+template <typename Run>
+int run_staged(HostPipe &pp, hipStream_t comp, StagedOperand o, StagedOperand i, bool preload_out,
+               unsigned howmany, Run run) {
+    if (pp.init()) return -1;
+    const int rc = run_staged_chunks(pp, comp, o, i, preload_out, howmany, run);
+    if (rc) {
+        (void)hipStreamSynchronize(pp.up);
+        (void)hipStreamSynchronize(pp.down);
+        (void)hipStreamSynchronize(comp);
+    }
+    return rc;
 }
 
 // Stockham twiddle table for a radix sequence, generated on the GPU in FP64.
@@ -1882,23 +1896,31 @@ mufft_hip_conv_batch *mufft_hip_create_conv_batch(const mufft_hip_conv_job *jobs
         d.out = static_cast<float2 *>(j.output);
     }
     // Twiddle tables per (log2 n, kind, direction) of the persistent kernel's variants.
+    // Transforms of 2^CB_SPLIT_LOG2 points use the 2^(CB_SPLIT_LOG2-1)-point
+    // sub-transform's table plus the split twiddles.
     struct TabKey {
         int logn, kind, dir;
-        float2 *tw, *rtw;
+        float2 *tw, *rtw, *stw;
     };
     std::vector<TabKey> tabs;
-    auto table_for = [&](int logn, int kind, int dir, float2 **tw, float2 **rtw) -> int {
+    auto table_for = [&](int logn, int kind, int dir, float2 **tw, float2 **rtw, float2 **stw) -> int {
         for (const auto &t : tabs)
             if (t.logn == logn && t.kind == kind && t.dir == dir) {
                 *tw = t.tw;
                 *rtw = t.rtw;
+                *stw = t.stw;
                 return 0;
             }
-        TabKey k{logn, kind, dir, nullptr, nullptr};
+        TabKey k{logn, kind, dir, nullptr, nullptr, nullptr};
         const int n = 1 << logn;
-        if (build_stockham_table(n, dir, fused_sequence(n, cb_points_per_thread(logn)), &k.tw, b->stream))
+        const int n2 = logn >= CB_SPLIT_LOG2 ? n / 2 : n;
+        if (build_stockham_table(n2, dir, fused_sequence(n2, cb_points_per_thread(logn)), &k.tw, b->stream))
             return -1;
         b->tables.push_back(k.tw);
+        if (n2 != n) {
+            if (build_split_table(n, n2, dir, &k.stw, b->stream)) return -1;
+            b->tables.push_back(k.stw);
+        }
         if (kind != KIND_C2C) {
             if (build_resolve_table(n, dir, &k.rtw, b->stream)) return -1;
             b->tables.push_back(k.rtw);
@@ -1906,6 +1928,7 @@ mufft_hip_conv_batch *mufft_hip_create_conv_batch(const mufft_hip_conv_job *jobs
         tabs.push_back(k);
         *tw = k.tw;
         *rtw = k.rtw;
+        *stw = k.stw;
         return 0;
     };
     // Tasks: runs of equal (method, N) jobs, split by workgroup capacity.
@@ -1953,14 +1976,20 @@ mufft_hip_conv_batch *mufft_hip_create_conv_batch(const mufft_hip_conv_job *jobs
                 if (sh.type == CB_FWD_SECOND && (m0 & MUFFT_CONV_METHOD_FLAG_ZERO_PAD_UPPER_HALF_SECOND))
                     t.flags |= RF_ZERO_PAD;
                 t.norm = 1.0f / (float)j0.N;
-                float2 *tw = nullptr, *rtw = nullptr;
-                if (table_for(t.logn, t.kind, t.dir, &tw, &rtw)) {
+                float2 *tw = nullptr, *rtw = nullptr, *stw = nullptr;
+                if (table_for(t.logn, t.kind, t.dir, &tw, &rtw, &stw)) {
                     destroy_batch(b);
                     return nullptr;
                 }
                 t.tw = tw;
                 t.rtw = rtw;
+                t.stw = stw;
+                t.k1 = 0;
                 (sh.type == CB_INV ? inv : fwd).push_back(t);
+                if (t.logn >= CB_SPLIT_LOG2) {   // second half of the split transform
+                    t.k1 = 1;
+                    (sh.type == CB_INV ? inv : fwd).push_back(t);
+                }
                 q = q2;
             }
         }
@@ -2100,9 +2129,13 @@ mufft_hip_ola *mufft_hip_create_ola(unsigned N, const float *taps, unsigned ntap
         return nullptr;
     }
     std::vector<float> hp(h, 0.0f);
-    if (hipMemcpy(hp.data(), taps, ntaps * sizeof(float), hipMemcpyDefault) != hipSuccess) {
-        (void)hipGetLastError();
-        memcpy(hp.data(), taps, ntaps * sizeof(float));  // plain host memory
+    const PtrInfo tp = classify_ptr(taps);
+    if (tp.kind == PTR_PAGEABLE) {
+        memcpy(hp.data(), taps, ntaps * sizeof(float));
+    } else if (hipMemcpy(hp.data(), taps, ntaps * sizeof(float), hipMemcpyDefault) != hipSuccess) {
+        set_err("copy of the filter taps failed");
+        mufft_hip_free_ola(o);
+        return nullptr;
     }
     DevBuf dh;
     hipStream_t st = o->fwd->stream;

@@ -10,6 +10,7 @@ namespace mufft {
 constexpr int CB_WG = 1024;           // threads per workgroup of the persistent kernel
 constexpr int CB_LDS_BYTES = 139264;  // 1024 threads x 17 padded points x 8 bytes
 constexpr int CB_MIN_LOG2 = 2;        // smallest fused transform inside the kernel (4 points)
+constexpr int CB_SPLIT_LOG2 = 15;     // 32768 points: two workgroups per transform
 
 enum CBTaskType : int { CB_FWD_FIRST = 0, CB_FWD_SECOND = 1, CB_INV = 2 };
 
@@ -32,6 +33,10 @@ struct CBTask {
     float norm;            // 1/N for the inverse multiply
     const float2 *tw;      // Stockham twiddles of the transform
     const float2 *rtw;     // real-transform resolve twiddles (r2c / c2r)
+    // Transforms of 2^CB_SPLIT_LOG2 points run as two workgroup tasks
+    // (k1 = 0, 1) of the split transform (split_impl.h); stw is its twiddle table.
+    int k1;
+    const float2 *stw;
 };
 
 // Tasks [0, nfwd) are the forward transforms, [nfwd, ntasks) the inverse
@@ -47,6 +52,6 @@ struct CBArgs {
 
 hipError_t launch_conv_batch(const CBArgs &a, int grid, hipStream_t s);
 int cb_jobs_per_task(int logn);
-int cb_points_per_thread(int logn);
+int cb_points_per_thread(int logn);   // of the (sub-)transform a task runs
 
 }  // namespace mufft

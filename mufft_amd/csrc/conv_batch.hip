@@ -19,12 +19,13 @@
 
 namespace mufft {
 
-// Fused-kernel variant used inside the persistent kernel (1024 threads):
-// 16 points per thread with split twiddles, 32 with the split exchange at
-// 32768 points.  Both fit CB_LDS_BYTES.
+// Fused-kernel variant used inside the persistent kernel (1024 threads): 16
+// points per thread with split twiddles; at 2^CB_SPLIT_LOG2 points the same
+// variant runs each 16384-point half of the split transform.  It fits CB_LDS_BYTES.
 MUFFT_HD constexpr int cb_variant(int N) {
-    return N >= 32768 ? rv_make(32, 1, 1) : rv_make(16, 0, 1);
+    return rv_make(16, 0, 1);
 }
+MUFFT_HD constexpr int cb_sub_n(int logn) { return logn >= CB_SPLIT_LOG2 ? (1 << (logn - 1)) : (1 << logn); }
 
 MUFFT_HD constexpr int cb_lds_bytes(int N) {
     return (CB_WG / rv_T(N, cb_variant(N))) * rv_lds_per(N, cb_variant(N)) *
@@ -32,15 +33,16 @@ MUFFT_HD constexpr int cb_lds_bytes(int N) {
 }
 
 int cb_jobs_per_task(int logn) {
-    const int N = 1 << logn;
+    const int N = cb_sub_n(logn);
     return CB_WG / rv_T(N, cb_variant(N));
 }
 
-int cb_points_per_thread(int logn) { return rv_E(1 << logn, cb_variant(1 << logn)); }
+int cb_points_per_thread(int logn) { return rv_E(cb_sub_n(logn), cb_variant(cb_sub_n(logn))); }
 
+// Jobs [t.job0, t.job0 + t.njobs) of a task, one fused transform each.
 template <int KIND, int DIR, int LOGN>
-__device__ __noinline__ void cb_run(const CBTask &t, const CBJob *__restrict__ jobs, int tid,
-                                       float2 *smem) {
+__device__ __forceinline__ void cb_run_rows(const CBTask &t, const CBJob *__restrict__ jobs, int tid,
+                                            float2 *smem) {
     constexpr int N = 1 << LOGN;
     constexpr int VAR = cb_variant(N);
     constexpr int T = rv_T(N, VAR);
@@ -66,6 +68,28 @@ __device__ __noinline__ void cb_run(const CBTask &t, const CBJob *__restrict__ j
     }
     row_transform<N, DIR, KIND, VAR>(X, Y, Xo, active, lt, t.tw, t.rtw, t.flags, t.norm,
                                       smem + sub * LDSN, reinterpret_cast<float *>(smem) + sub * LDSN);
+}
+
+// Half t.k1 of one job's transform over two workgroups (split_impl.h).
+template <int KIND, int DIR, int LOGN>
+__device__ __forceinline__ void cb_run_split(const CBTask &t, const CBJob *__restrict__ jobs, int tid,
+                                             float2 *smem) {
+    constexpr int N2 = 1 << (LOGN - 1);
+    constexpr int VAR = cb_variant(N2);
+    static_assert(cb_lds_bytes(N2) <= CB_LDS_BYTES, "LDS budget of the persistent kernel");
+    const CBJob &j = jobs[t.job0];
+    const float2 *X = t.type == CB_FWD_FIRST ? j.first : (t.type == CB_FWD_SECOND ? j.second : j.spec1);
+    const float2 *Y = t.type == CB_INV ? j.spec2 : nullptr;
+    float2 *Xo = t.type == CB_FWD_FIRST ? j.spec1 : (t.type == CB_FWD_SECOND ? j.spec2 : j.out);
+    split_transform<N2, 2, DIR, KIND, VAR>(X, Y, Xo, t.k1, tid, t.tw, t.stw, t.rtw, t.flags, t.norm, smem,
+                                           reinterpret_cast<float *>(smem));
+}
+
+template <int KIND, int DIR, int LOGN>
+__device__ __noinline__ void cb_run(const CBTask &t, const CBJob *__restrict__ jobs, int tid,
+                                       float2 *smem) {
+    if constexpr (LOGN >= CB_SPLIT_LOG2) cb_run_split<KIND, DIR, LOGN>(t, jobs, tid, smem);
+    else cb_run_rows<KIND, DIR, LOGN>(t, jobs, tid, smem);
 }
 
 template <int KIND, int DIR, int LO, int HI>

@@ -24,8 +24,14 @@
 
 namespace mufft {
 
+// Workgroup k1 of one transform: X / Y / Xo are the transform's input, second
+// operand and output, L / Lf its LDS (rv_lds_per(N2, VAR) slots).  All T
+// threads of the workgroup call it.
 template <int N2, int N1, int DIR, int KIND, int VAR>
-__global__ void __launch_bounds__(rv_T(N2, VAR), rv_waves_per_simd(N2, VAR)) fft_split_kernel(RowArgs a) {
+__device__ __forceinline__ void split_transform(const float2 *X, const float2 *Y, float2 *Xo, int k1,
+                                                int lt, const float2 *tw, const float2 *stw,
+                                                const float2 *rtw, unsigned flags, float norm,
+                                                float2 *L, float *Lf) {
     constexpr int E = rv_E(N2, VAR);
     constexpr int LOGE = ilog2(E);
     constexpr int T = rv_T(N2, VAR);
@@ -33,35 +39,25 @@ __global__ void __launch_bounds__(rv_T(N2, VAR), rv_waves_per_simd(N2, VAR)) fft
     constexpr int TWM = rv_twm(VAR);
     constexpr bool SPLIT = XM == XM_SPLIT;
     constexpr int NT = N1 * N2;
-    constexpr int LDSN = rv_lds_per(N2, VAR);
     static_assert(rv_B(N2, VAR) == 1, "one sub-transform per workgroup");
     static_assert(N1 == 2 || N1 == 4, "split factor");
     static_assert(KIND == KIND_C2C || N1 == 2, "real transforms split in two");
-    __shared__ float2 smem[SPLIT ? (LDSN + 1) / 2 : LDSN];
-    float2 *L = smem;
-    float *Lf = reinterpret_cast<float *>(smem);
     auto slot = [](int o) { return XM == XM_FLAT ? o : pad_slot<LOGE>(o); };
+    k1 = __builtin_amdgcn_readfirstlane(k1);   // uniform: it feeds scalar buffer offsets
 
-    const unsigned g = xcd_block(blockIdx.x, gridDim.x);
-    const long long t = g / N1;
-    const int k1 = (int)(g % N1);
-    const int lt = threadIdx.x;
-    const unsigned flags = a.flags;
     const bool zp = (flags & RF_ZERO_PAD) != 0;
     const bool mul = (flags & RF_MUL) != 0;
     const bool cj = (flags & RF_CONJ2) != 0;
     const bool full = (flags & RF_FULL) != 0;
     const bool adj = (flags & (RF_ADJ_R2C | RF_ADJ_C2R)) != 0;
-    const float norm = a.norm;
 
     const unsigned nin = KIND == KIND_C2R ? NT + 1 : ((zp && !mul) ? NT / 2 : NT);
     const unsigned nout = KIND == KIND_R2C ? (full ? 2 * NT : NT + 1) : NT;
-    const __amdgpu_buffer_rsrc_t rin = make_rsrc(a.in + t * a.idist, nin * 8u);
-    const __amdgpu_buffer_rsrc_t rin2 =
-        make_rsrc(a.in2 + t * a.idist2, mul ? (KIND == KIND_C2R ? NT + 1 : NT) * 8u : 0u);
-    const __amdgpu_buffer_rsrc_t rout = make_rsrc(a.out + t * a.odist, nout * 8u);
-    const __amdgpu_buffer_rsrc_t rrtw = make_rsrc(a.rtw, KIND == KIND_C2C ? 0u : NT * 8u);
-    const __amdgpu_buffer_rsrc_t rstw = make_rsrc(a.stw, (N1 - 1) * N2 * 8u);
+    const __amdgpu_buffer_rsrc_t rin = make_rsrc(X, nin * 8u);
+    const __amdgpu_buffer_rsrc_t rin2 = make_rsrc(Y, mul ? (KIND == KIND_C2R ? NT + 1 : NT) * 8u : 0u);
+    const __amdgpu_buffer_rsrc_t rout = make_rsrc(Xo, nout * 8u);
+    const __amdgpu_buffer_rsrc_t rrtw = make_rsrc(rtw, KIND == KIND_C2C ? 0u : NT * 8u);
+    const __amdgpu_buffer_rsrc_t rstw = make_rsrc(stw, (N1 - 1) * N2 * 8u);
     auto ld = [](__amdgpu_buffer_rsrc_t r, int vp, int sp) {
         return bload(r, (unsigned)vp * 8u, (unsigned)sp * 8u);
     };
@@ -119,7 +115,7 @@ __global__ void __launch_bounds__(rv_T(N2, VAR), rv_waves_per_simd(N2, VAR)) fft
     });
 
     constexpr int RAD = rv_R(N2, VAR);
-    stockham_passes<N2, E, DIR, XM, TWM, RAD>(v, lt, TwBuffer{make_rsrc(a.tw, (N2 - RAD) * 8u)}, L, Lf,
+    stockham_passes<N2, E, DIR, XM, TWM, RAD>(v, lt, TwBuffer{make_rsrc(tw, (N2 - RAD) * 8u)}, L, Lf,
                                               slot);
 
     // v[c] now holds X[N1*k2 + k1] for k2 = lt + T*c.
@@ -168,6 +164,18 @@ __global__ void __launch_bounds__(rv_T(N2, VAR), rv_waves_per_simd(N2, VAR)) fft
             }
         });
     }
+}
+
+template <int N2, int N1, int DIR, int KIND, int VAR>
+__global__ void __launch_bounds__(rv_T(N2, VAR), rv_waves_per_simd(N2, VAR)) fft_split_kernel(RowArgs a) {
+    constexpr int LDSN = rv_lds_per(N2, VAR);
+    __shared__ float2 smem[rv_xm(VAR) == XM_SPLIT ? (LDSN + 1) / 2 : LDSN];
+    const unsigned g = xcd_block(blockIdx.x, gridDim.x);
+    const long long t = g / N1;
+    const int k1 = (int)(g % N1);
+    split_transform<N2, N1, DIR, KIND, VAR>(a.in + t * a.idist, a.in2 + t * a.idist2, a.out + t * a.odist,
+                                            k1, threadIdx.x, a.tw, a.stw, a.rtw, a.flags, a.norm, smem,
+                                            reinterpret_cast<float *>(smem));
 }
 
 }  // namespace mufft

@@ -1,6 +1,8 @@
-"""The uneven convolution batch (one persistent launch, device work queue,
-hipGraph replay) against FP64 convolution, direct O(N^2) convolution at small
-N, and the per-plan drop-in path on the same jobs."""
+"""The uneven convolution batch (two persistent launches, forward then inverse
+tasks, each pulling from a device work queue; hipGraph replay) against FP64
+convolution, direct O(N^2) convolution at small N, and the per-plan drop-in
+path on the same jobs.  Mono N = 65536 and stereo N = 32768 jobs run their
+32768-point transforms as two-workgroup split tasks."""
 import numpy as np
 import pytest
 
@@ -59,7 +61,7 @@ def test_uneven_batch_graph_and_direct(gpu):
     for j in jobs:
         j[4].zero_()
     torch.cuda.synchronize()
-    b.execute()                     # second replay: counters and queue were reset inside the graph
+    b.execute()                     # second replay: queue heads were reset inside the graph
     assert b.status() == 0
     for j, f in zip(jobs, first):
         np.testing.assert_array_equal(from_dev(j[4]).view(np.uint32), f.view(np.uint32))
