@@ -112,8 +112,10 @@ def cpu_baseline(x_c2c: np.ndarray, x_real: np.ndarray, budget_s: float = 12.0):
 def conv_batch_bench(dev, stream, reps: int = 10):
     """Uneven convolution batch: 512 mono jobs, N = 2^(8 + u mod 9) real, signal
     N/2 uniform, filter N/2 taps z_n*exp(-n/(N/8)), z ~ N(0,1); plus 128 stereo
-    jobs N = 16384 (complex signal N/2).  One persistent graph launch versus the
-    per-plan drop-in path (plans cached per N, 3 synchronous calls per job)."""
+    jobs N = 16384 (complex signal N/2).  One graph launch (reset + forward and
+    inverse persistent phases) versus the per-plan drop-in path (plans cached
+    per N, 3 synchronous calls per job); the same kernels launched without the
+    graph are reported too."""
     import torch
 
     import mufft_amd as M
