@@ -205,6 +205,62 @@ def conv_batch_bench(dev, stream, reps: int = 10):
     }
 
 
+def two_d_bench(dev, stream, reps: int = 10):
+    """The 2D configurations: c2c 1024 x 1024 x 16 images forward, r2c and c2r
+    2048 x 2048 x 4 images; pixels uniform[0,1) + N(0, 0.1^2).  Algorithmic
+    bytes count one read and one write of each image (the Nx/2+1 meaningful
+    columns of the padded real layout)."""
+    import torch
+
+    import mufft_amd as M
+    from oracle import fp64
+
+    def timed(fn):
+        with torch.cuda.stream(stream):
+            for _ in range(2):
+                fn()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(reps):
+                fn()
+            e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps * 1e-3
+
+    g = torch.Generator(device=dev)
+    g.manual_seed(0x3001)
+    out = {}
+    n, b = 1024, 16
+    x = torch.complex(torch.rand((b, n, n), generator=g, device=dev) + 0.1 * torch.randn((b, n, n), generator=g, device=dev),
+                      torch.rand((b, n, n), generator=g, device=dev) + 0.1 * torch.randn((b, n, n), generator=g, device=dev))
+    y = torch.empty_like(x)
+    p = M.Plan2D("c2c", n, n, M.FORWARD)
+    t = timed(lambda: p.execute_batched(y, x, b, stream=stream))
+    alg = 16.0 * n * n * b
+    err = fp64.rel_rms(y[0].cpu().numpy(), np.fft.fft2(x[0].cpu().numpy().astype(np.complex128)))
+    out["c2c_1024x1024x16"] = {"us": round(t * 1e6, 1), "hbm_frac": round(alg / t / 8e12, 4),
+                               "rel_rms_vs_fp64": err}
+    del x, y
+    n, b = 2048, 4
+    g.manual_seed(0x3002)
+    xr = torch.rand((b, n, n), generator=g, device=dev) + 0.1 * torch.randn((b, n, n), generator=g, device=dev)
+    yc = torch.empty((b, n, n), dtype=torch.complex64, device=dev)  # padded rows of n complex
+    zr = torch.empty_like(xr)
+    pr = M.Plan2D("r2c", n, n, M.FORWARD)
+    pc = M.Plan2D("c2r", n, n, M.INVERSE)
+    alg = b * n * (4.0 * n + 8.0 * (n // 2 + 1))
+    t_r = timed(lambda: pr.execute_batched(yc, xr, b, stream=stream))
+    t_c = timed(lambda: pc.execute_batched(zr, yc, b, stream=stream))
+    err = fp64.rel_rms(yc[0, :, : n // 2 + 1].cpu().numpy(), np.fft.rfft2(xr[0].cpu().numpy().astype(np.float64)))
+    rt = fp64.rel_rms(zr[0].cpu().numpy() / (n * n), xr[0].cpu().numpy())
+    out["r2c_2048x2048x4"] = {"us": round(t_r * 1e6, 1), "hbm_frac": round(alg / t_r / 8e12, 4),
+                              "rel_rms_vs_fp64": err}
+    out["c2r_2048x2048x4"] = {"us": round(t_c * 1e6, 1), "hbm_frac": round(alg / t_c / 8e12, 4),
+                              "roundtrip_rel_rms": rt}
+    return out
+
+
 def host_paths_bench():
     """PCIe-inclusive rates of the host-memory paths (not part of `value`):
     one drop-in call on pinned buffers (used in place) and on pageable buffers
@@ -398,6 +454,7 @@ def main():
     conv = conv_batch_bench(dev, stream)
     dconv = diff_conv_bench(dev, stream)
     host = host_paths_bench()
+    twod = two_d_bench(dev, stream)
 
     out = {
         "metric": METRIC,
@@ -441,6 +498,7 @@ def main():
         "conv_batch": conv,
         "diff_conv": dconv,
         "host_paths": host,
+        "two_d": twod,
         "input_gen_s": round(gen_s, 2),
     }
     print(json.dumps(out))
