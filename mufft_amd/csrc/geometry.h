@@ -34,44 +34,24 @@ enum RowFlag : unsigned {
 MUFFT_HD constexpr int ilog2(long long n) { return n <= 1 ? 0 : 1 + ilog2(n >> 1); }
 
 // ---------------------------------------------------------------------------
-// Fused single-workgroup transform of N complex points (N = 2 .. 32768).
-// Each thread owns E points in registers, T = N/E threads cooperate on one
-// transform, B transforms share a workgroup.  Passes: NP-1 passes of radix E
-// and a last pass of radix N / E^(NP-1).  Between passes the points move
-// through LDS with one padding slot every E points (bank-conflict avoidance);
-// at N >= 32768 the exchange goes through LDS in two halves (real parts, then
-// imaginary parts) so the whole transform fits one workgroup.
+// Variants of the fused row kernel, one int of bit fields:
+//   bits 0-7   E, points per thread (clamped to the transform length);
+//   bits 8-11  LDS exchange mode (XMode in stockham.h: 0 padded complex,
+//              1 split real/imaginary, 2 unpadded complex);
+//   bits 12-15 twiddle mode (TwMode: 0 every twiddle from the table, 1 split
+//              product);
+//   bits 16-19 log2 N1: one transform split over N1 workgroups
+//              (split_impl.h); E, T, the pass count and the LDS size then
+//              refer to the N/N1-point sub-transform;
+//   bits 20-23 workgroups per CU to budget registers for (0: no request);
+//   bits 24-27 log2 of the radix of every pass but the last (0: radix E);
+//              a radix below E gives each thread E/radix butterflies per pass.
+// The planner's static choice is rv_default(); the autotuner may pick any
+// registered variant.
 // ---------------------------------------------------------------------------
-MUFFT_HD constexpr int row_E(int N) { return N <= 16 ? N : (N <= 4096 ? 16 : 32); }
-MUFFT_HD constexpr int row_T(int N) { return N / row_E(N); }
-MUFFT_HD constexpr int row_B(int N) { return row_T(N) >= 256 ? 1 : 256 / row_T(N); }
-MUFFT_HD constexpr int row_WG(int N) { return row_T(N) * row_B(N); }
-MUFFT_HD constexpr int row_NP(int N) {
-    return (ilog2(N) + ilog2(row_E(N)) - 1) / ilog2(row_E(N));
-}
-MUFFT_HD constexpr bool row_split(int N) { return N >= 32768; }
-MUFFT_HD constexpr int row_lds_per(int N) { return N + N / row_E(N); }  // padded points
-
-// ---------------------------------------------------------------------------
-// Variants of the fused row kernel.  A variant fixes E (points per thread,
-// clamped to N), the LDS exchange mode (XMode in stockham.h: 0 padded complex,
-// 1 split real/imaginary, 2 unpadded complex) and the twiddle mode (TwMode:
-// 0 every twiddle from the table, 1 split product).  The planner's static
-// choice is rv_default(); the autotuner may pick any registered variant.
-// ---------------------------------------------------------------------------
-//
-// A variant may also split one transform over N1 = 2^s workgroups (bits 16-19
-// hold s; split_impl.h): each workgroup then runs an N/N1-point transform, and
-// E, T, the pass count and the LDS size below refer to that sub-transform.
 MUFFT_HD constexpr int rv_make(int E, int xm, int twm) { return E | (xm << 8) | (twm << 12); }
-//
-// Bits 20-23 ask for that many workgroups per CU (0: no request): the kernel
-// is then compiled with a register budget that lets them co-reside.
 MUFFT_HD constexpr int rv_make_split(int log2n1, int var) { return var | (log2n1 << 16); }
 MUFFT_HD constexpr int rv_with_occ(int var, int wgs_per_cu) { return var | (wgs_per_cu << 20); }
-// Bits 24-27: log2 of the radix of every pass but the last (0: radix E).  A
-// radix below E gives each thread E / radix butterflies per pass: more passes,
-// fewer registers live inside one butterfly.
 MUFFT_HD constexpr int rv_with_radix(int var, int log2r) { return var | (log2r << 24); }
 MUFFT_HD constexpr int rv_split(int var) { return (var >> 16) & 0xf; }   // log2 N1
 MUFFT_HD constexpr int rv_occ(int var) { return (var >> 20) & 0xf; }
@@ -100,9 +80,6 @@ MUFFT_HD constexpr int rv_NP(int N, int var) {
 MUFFT_HD constexpr int rv_lds_per(int N, int var) {
     return rv_xm(var) == 2 ? rv_n(N, var) : rv_n(N, var) + rv_n(N, var) / rv_E(N, var);
 }
-MUFFT_HD constexpr long long rv_lds_bytes(int N, int var) {
-    return (long long)rv_B(N, var) * rv_lds_per(N, var) * (rv_xm(var) == 1 ? 4 : 8);
-}
 // Static choice per size and transform kind (c2c, or r2c/c2r whose fused
 // transform has N = half the real length), from sweeps of all registered
 // variants on MI355X at 2^27-2^28 points per launch: split-product twiddles
@@ -128,7 +105,8 @@ MUFFT_HD constexpr int seq_p(int E, int s) { return 1 << (ilog2(E) * s); }
 // adjacent columns.  Lanes run along the columns (unit stride) so every row
 // segment of a tile is one coalesced access.
 // ---------------------------------------------------------------------------
-MUFFT_HD constexpr int col_E(int L) { return row_E(L); }
+// Points per thread: all of a tiny column, 16 up to 4096 points, 32 beyond.
+MUFFT_HD constexpr int col_E(int L) { return L <= 16 ? L : (L <= 4096 ? 16 : 32); }
 MUFFT_HD constexpr int col_T(int L) { return L / col_E(L); }
 // Tile width of the four-step column kernel (whose transposed store needs the
 // full complex tile in LDS).
@@ -144,7 +122,6 @@ MUFFT_HD constexpr int col_TX(int L) { return L == 1024 ? 16 : (L == 2048 ? 4 : 
 MUFFT_HD constexpr int col_xm(int L) { return (L == 1024 || L == 2048) ? 1 : 0; }    // XMode
 MUFFT_HD constexpr int col_twm(int L) { return (L == 1024 || L == 2048) ? 1 : 0; }   // TwMode
 MUFFT_HD constexpr int col_WG(int L) { return col_T(L) * col_TX(L); }
-MUFFT_HD constexpr int col_NP(int L) { return row_NP(L); }
 MUFFT_HD constexpr int col_lds_rows(int L) { return L + L / col_E(L); }
 constexpr int COL_MAX_LOG2 = 13;   // fused column kernel covers L <= 8192
 constexpr int ROW_MAX_LOG2 = 16;   // fused row kernels (one or two workgroups per transform) cover N <= 65536
