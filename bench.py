@@ -391,6 +391,21 @@ def main():
     dcal = torch.empty_like(dx)
     dcal.copy_(dx)
     torch.cuda.synchronize()
+    # Achievable streaming bandwidth: the library's copy kernel (the FFT kernels'
+    # 8-byte accesses and non-temporal policy) on the same 512 MiB.
+    from mufft_amd import _lib
+    L = _lib.load()
+    nbytes = dx.numel() * 8
+    for _ in range(3):
+        L.mufft_hip_stream_copy(dcal.data_ptr(), dx.data_ptr(), nbytes, None)
+    c0 = torch.cuda.Event(enable_timing=True)
+    c1 = torch.cuda.Event(enable_timing=True)
+    c0.record()
+    for _ in range(10):
+        assert L.mufft_hip_stream_copy(dcal.data_ptr(), dx.data_ptr(), nbytes, None) == 0
+    c1.record()
+    torch.cuda.synchronize()
+    copy_gbs = 2.0 * nbytes / (c0.elapsed_time(c1) / 10 * 1e-3) / 1e9
     del dcal
 
     # Plan-time autotuning (outside the timed region, like the reference's
@@ -485,6 +500,8 @@ def main():
             "traffic": traffic_from_profiles(),
             "kernel_ms": round(kern_ms, 5),
             "alg_bytes_per_launch": int(alg_bytes),
+            "stream_copy_GBs": round(copy_gbs, 1),
+            "frac_of_stream_copy": round(achieved / copy_gbs, 4),
         },
         "cpu_baseline": cpu,
         "accuracy": {

@@ -141,6 +141,11 @@ int mufft_hip_conv_batch_status(mufft_hip_conv_batch *batch);
 int mufft_hip_conv_batch_info(const mufft_hip_conv_batch *batch, int *ntasks, int *grid);
 void mufft_hip_free_conv_batch(mufft_hip_conv_batch *batch);
 
+/* Device-to-device copy with the FFT kernels' access width and streaming
+ * cache policy (bytes a multiple of 8): the achievable-bandwidth reference for
+ * their roofline figures.  No counterpart in the reference. */
+int mufft_hip_stream_copy(void *dst, const void *src, size_t bytes, void *stream);
+
 /* ---- Streaming overlap-add FIR filtering ----
  * y = x * h for an unbounded stream x, hop N/2, FFT size N (power of two,
  * 4..65536), real filter of ntaps <= N/2 taps (host or device pointer, copied).

@@ -61,6 +61,7 @@ SIGNATURES = [
     ("mufft_hip_execute_plan_1d_mul_batched", _i,
      [_vp, _vp, _vp, _vp, _u, _sz, _sz, _sz, _f, _i, _i, _vp]),
     ("mufft_hip_mul_reduce", _i, [_vp, _vp, _vp, _u, _u, _sz, _sz, _f, _i, _vp]),
+    ("mufft_hip_stream_copy", _i, [_vp, _vp, ctypes.c_size_t, _vp]),
     ("mufft_hip_create_ola", _vp, [_u, _vp, _u]),
     ("mufft_hip_execute_ola", _i, [_vp, _vp, _vp, _u, _vp]),
     ("mufft_hip_reset_ola", _i, [_vp, _vp]),

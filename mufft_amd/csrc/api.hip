@@ -1015,6 +1015,17 @@ int mufft_hip_mul_reduce(void *output, const void *a, const void *b, unsigned n,
     return 0;
 }
 
+int mufft_hip_stream_copy(void *dst, const void *src, size_t bytes, void *stream) {
+    if (!dst || !src || (bytes & 7)) {
+        set_err("null buffer or size not a multiple of 8 bytes");
+        return -1;
+    }
+    if (kernel_ptrs(&dst, &src)) return -1;
+    HIP_TRY(launch_stream_copy(static_cast<const float2 *>(src), static_cast<float2 *>(dst),
+                               (long long)(bytes / 8), static_cast<hipStream_t>(stream)));
+    return 0;
+}
+
 int mufft_hip_plan_1d_launches(const mufft_plan_1d *p) {
     if (!p) return -1;
     if (p->eng.row_fused()) return 1;

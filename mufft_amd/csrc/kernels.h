@@ -120,6 +120,7 @@ hipError_t launch_mul(const MulArgs &a, hipStream_t s);
 hipError_t launch_fold(const MulArgs &a, hipStream_t s);
 hipError_t launch_mul_reduce(const MulArgs &a, hipStream_t s);
 hipError_t launch_four_step_cols(int logN1, int direction, const FourStepArgs &a, hipStream_t s);
+hipError_t launch_stream_copy(const float2 *in, float2 *out, long long n, hipStream_t s);
 hipError_t launch_ola(const float *y, float *out, const float *tail_in, float *tail_out,
                       long long n, long long nblocks, hipStream_t s);
 hipError_t launch_twiddles(const TwiddleArgs &a, hipStream_t s);

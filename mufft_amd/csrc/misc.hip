@@ -271,6 +271,34 @@ hipError_t launch_ola(const float *y, float *out, const float *tail_in, float *t
     return hipGetLastError();
 }
 
+// Streaming copy with the row kernels' access width and cache policy (8 bytes
+// per lane, non-temporal): the achievable-HBM reference the FFT kernels are
+// compared with.  Each lane moves 16 elements per step.
+__global__ void __launch_bounds__(256) stream_copy_kernel(const u32x2 *__restrict__ in,
+                                                          u32x2 *__restrict__ out, long long n) {
+    constexpr int PER = 16;
+    const long long chunk = 256LL * PER;
+    for (long long base = (long long)blockIdx.x * chunk; base < n; base += (long long)gridDim.x * chunk) {
+        u32x2 v[PER];
+        for (int e = 0; e < PER; ++e) {
+            const long long i = base + threadIdx.x + 256LL * e;
+            v[e] = i < n ? __builtin_nontemporal_load(in + i) : u32x2{0u, 0u};
+        }
+        for (int e = 0; e < PER; ++e) {
+            const long long i = base + threadIdx.x + 256LL * e;
+            if (i < n) __builtin_nontemporal_store(v[e], out + i);
+        }
+    }
+}
+
+hipError_t launch_stream_copy(const float2 *in, float2 *out, long long n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const long long blocks = (n + 4095) / 4096;
+    hipLaunchKernelGGL(stream_copy_kernel, dim3((unsigned)(blocks < 16384 ? blocks : 16384)), dim3(256), 0,
+                       s, reinterpret_cast<const u32x2 *>(in), reinterpret_cast<u32x2 *>(out), n);
+    return hipGetLastError();
+}
+
 hipError_t launch_fold(const MulArgs &a, hipStream_t s) {
     const long long work = (a.n / 2 + 1) * a.batch;
     if (work <= 0) return hipSuccess;

@@ -127,3 +127,18 @@ def test_stream_ordered_extensions_refuse_pageable(gpu):
     dx = to_dev(x, gpu)
     with pytest.raises(M.MufftError):
         p.execute_mul_batched(y, dx, dx, 1)
+
+
+def test_stream_copy(gpu):
+    """The streaming copy used as the bandwidth reference copies exactly and
+    refuses sizes that are not a multiple of 8 bytes."""
+    import torch
+    from mufft_amd import _lib
+    L = _lib.load()
+    for n in (1, 4095, 4096, 1 << 20):
+        x = torch.randn(n, dtype=torch.complex64, device=gpu)
+        y = torch.zeros_like(x)
+        assert L.mufft_hip_stream_copy(y.data_ptr(), x.data_ptr(), n * 8, None) == 0
+        torch.cuda.synchronize()
+        assert torch.equal(torch.view_as_real(x), torch.view_as_real(y))
+    assert L.mufft_hip_stream_copy(y.data_ptr(), x.data_ptr(), 12, None) != 0
