@@ -37,6 +37,9 @@ __device__ __forceinline__ void row_transform(const float2 *X, const float2 *Y, 
     constexpr int RAD = rv_R(N, VAR);
     constexpr bool SPLIT = XM == XM_SPLIT;
     constexpr bool BUF = T >= 64;
+    // A transform of at most 64 threads lies in one wave (its threads are
+    // consecutive): its exchanges need no workgroup barrier.
+    constexpr bool WAVE = T <= 64;
     auto slot = [](int o) { return XM == XM_FLAT ? o : pad_slot<LOGE>(o); };
 
     const bool zp = (flags & RF_ZERO_PAD) != 0;
@@ -134,9 +137,9 @@ __device__ __forceinline__ void row_transform(const float2 *X, const float2 *Y, 
 
     if constexpr (BUF) {
         const unsigned ntw = rv_NP(N, VAR) > 1 ? (unsigned)(N - RAD) : 0u;
-        stockham_passes<N, E, DIR, XM, TWM, RAD>(v, lt, TwBuffer{make_rsrc(tw, ntw * 8u)}, L, Lf, slot);
+        stockham_passes<N, E, DIR, XM, TWM, RAD, WAVE>(v, lt, TwBuffer{make_rsrc(tw, ntw * 8u)}, L, Lf, slot);
     } else {
-        stockham_passes<N, E, DIR, XM, TWM, RAD>(v, lt, TwGlobal{tw}, L, Lf, slot);
+        stockham_passes<N, E, DIR, XM, TWM, RAD, WAVE>(v, lt, TwGlobal{tw}, L, Lf, slot);
     }
 
     if constexpr (KIND == KIND_R2C) {
@@ -161,7 +164,7 @@ __device__ __forceinline__ void row_transform(const float2 *X, const float2 *Y, 
             constexpr int c = decltype(C)::value;
             Lc[slot(lt + T * c - N / 2)] = v[c];
         });
-        __syncthreads();
+        exchange_sync<WAVE>();
         if (BUF || active) {
             static_for<0, E / 2>([&](auto C) {
                 constexpr int c = decltype(C)::value;

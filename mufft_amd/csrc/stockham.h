@@ -84,12 +84,30 @@ MUFFT_HD constexpr int np_for(int N, int E) {
 // over its points v[j + m*J].  `twl` reads the twiddle table of the sequence
 // (R, ..., R, r_last).  `Slot(o)` maps a point index to its LDS slot; `L` is
 // the complex LDS view, `Lf` the float view used by the split exchange.
-template <int N, int E, int DIR, int XM, int TWM, int R = E, typename TwLoad, typename Slot>
+// Orders the LDS exchange between the threads of one transform.  When they
+// all sit in one wavefront (WAVE), the wave's own LDS accesses are already
+// executed in order, so only the compiler must not move LDS accesses across
+// the point (wavefront-scope fences): no workgroup barrier, and the other
+// waves of the workgroup run on.
+template <bool WAVE>
+__device__ __forceinline__ void exchange_sync() {
+    if constexpr (WAVE) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
+template <int N, int E, int DIR, int XM, int TWM, int R = E, bool WAVE = false, typename TwLoad,
+          typename Slot>
 __device__ __forceinline__ void stockham_passes(float2 (&v)[E], int lt, TwLoad twl,
                                                 float2 *L, float *Lf, Slot slot) {
     constexpr int T = N / E;
     constexpr int NP = np_for(N, R);
     static_assert(R <= E && E % R == 0, "pass radix divides the points per thread");
+    static_assert(!WAVE || T <= 64, "wave-local exchange needs the transform within one wave");
     static_for<0, NP>([&](auto S) {
         constexpr int s = decltype(S)::value;
         constexpr int r = seq_radix(N, R, NP, s);
@@ -123,27 +141,27 @@ __device__ __forceinline__ void stockham_passes(float2 (&v)[E], int lt, TwLoad t
             };
             if constexpr (XM != XM_SPLIT) {
                 publish([](float2 z) { return z; }, [&](int o, float2 z) { L[o] = z; });
-                __syncthreads();
+                exchange_sync<WAVE>();
                 static_for<0, E>([&](auto C) {
                     constexpr int c = decltype(C)::value;
                     v[c] = L[slot(lt + T * c)];
                 });
-                __syncthreads();
+                exchange_sync<WAVE>();
             } else {
                 publish([](float2 z) { return z.x; }, [&](int o, float x) { Lf[o] = x; });
-                __syncthreads();
+                exchange_sync<WAVE>();
                 static_for<0, E>([&](auto C) {
                     constexpr int c = decltype(C)::value;
                     v[c].x = Lf[slot(lt + T * c)];
                 });
-                __syncthreads();
+                exchange_sync<WAVE>();
                 publish([](float2 z) { return z.y; }, [&](int o, float y) { Lf[o] = y; });
-                __syncthreads();
+                exchange_sync<WAVE>();
                 static_for<0, E>([&](auto C) {
                     constexpr int c = decltype(C)::value;
                     v[c].y = Lf[slot(lt + T * c)];
                 });
-                __syncthreads();
+                exchange_sync<WAVE>();
             }
         }
     });
