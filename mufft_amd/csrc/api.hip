@@ -43,6 +43,21 @@ void set_err(const char *fmt, ...) {
     va_end(ap);
 }
 
+// C entry points that build host containers run their body through this, so
+// an exception (std::bad_alloc) becomes an error return instead of crossing
+// the C ABI.
+template <typename R, typename F>
+R guarded(R fail, F &&body) {
+    try {
+        return body();
+    } catch (const std::exception &e) {
+        set_err("host exception: %s", e.what());
+    } catch (...) {
+        set_err("host exception");
+    }
+    return fail;
+}
+
 #define HIP_TRY(expr)                                                              \
     do {                                                                           \
         hipError_t e_ = (expr);                                                    \
@@ -671,7 +686,7 @@ static void destroy_1d(mufft_plan_1d *p) {
     delete p;
 }
 
-static mufft_plan_1d *create_1d(int kind, unsigned Nc, int direction, unsigned flags) {
+static mufft_plan_1d *create_1d_impl(int kind, unsigned Nc, int direction, unsigned flags) {
     if (!is_pow2(Nc) || Nc < 2 || Nc > (1u << 27)) {
         set_err("invalid transform size");
         return nullptr;
@@ -721,6 +736,10 @@ static mufft_plan_1d *create_1d(int kind, unsigned Nc, int direction, unsigned f
         break;
     }
     return p;
+}
+
+static mufft_plan_1d *create_1d(int kind, unsigned Nc, int direction, unsigned flags) {
+    return guarded<mufft_plan_1d *>(nullptr, [&] { return create_1d_impl(kind, Nc, direction, flags); });
 }
 
 // Distances in complex elements; extra = RF_MUL / RF_CONJ2 for fused multiplies.
@@ -1071,7 +1090,7 @@ static void destroy_2d(mufft_plan_2d *p) {
     delete p;
 }
 
-static mufft_plan_2d *create_2d(int kind, unsigned Nx_c, unsigned Ny, int direction,
+static mufft_plan_2d *create_2d_impl(int kind, unsigned Nx_c, unsigned Ny, int direction,
                                 unsigned flags) {
     if (!is_pow2(Nx_c) || !is_pow2(Ny) || Nx_c < 2 || Ny < 2 || Nx_c > (1u << 20) ||
         Ny > (1u << 20)) {
@@ -1120,6 +1139,11 @@ static mufft_plan_2d *create_2d(int kind, unsigned Nx_c, unsigned Ny, int direct
         p->out_bytes = px * 8;         // Nx_real * Ny floats
     }
     return p;
+}
+
+static mufft_plan_2d *create_2d(int kind, unsigned Nx_c, unsigned Ny, int direction,
+                                unsigned flags) {
+    return guarded<mufft_plan_2d *>(nullptr, [&] { return create_2d_impl(kind, Nx_c, Ny, direction, flags); });
 }
 
 // howmany packed images.
@@ -1271,7 +1295,7 @@ void mufft_free_plan_conv(mufft_plan_conv *plan) {
     delete plan;
 }
 
-mufft_plan_conv *mufft_create_plan_conv(unsigned N, unsigned flags, unsigned method) {
+static mufft_plan_conv *mufft_create_plan_conv_impl(unsigned N, unsigned flags, unsigned method) {
     if (!is_pow2(N) || N < 4) {
         set_err("invalid transform size");
         return nullptr;
@@ -1302,6 +1326,11 @@ mufft_plan_conv *mufft_create_plan_conv(unsigned N, unsigned flags, unsigned met
         return nullptr;
     }
     return c;
+}
+
+
+mufft_plan_conv *mufft_create_plan_conv(unsigned N, unsigned flags, unsigned method) {
+    return guarded<mufft_plan_conv *>(nullptr, [&] { return mufft_create_plan_conv_impl(N, flags, method); });
 }
 
 size_t mufft_conv_get_transformed_block_size(mufft_plan_conv *plan) {
@@ -1497,7 +1526,7 @@ int mufft_hip_synchronize(void *stream) {
     return 0;
 }
 
-int mufft_hip_twiddle_table(unsigned N, int direction, float *host_out, unsigned count) {
+static int mufft_hip_twiddle_table_impl(unsigned N, int direction, float *host_out, unsigned count) {
     if (!is_pow2(N) || N < 2 || N > (1u << ROW_MAX_LOG2)) {
         set_err("invalid transform size");
         return -1;
@@ -1520,6 +1549,11 @@ int mufft_hip_twiddle_table(unsigned N, int direction, float *host_out, unsigned
         return -1;
     }
     return size;
+}
+
+
+int mufft_hip_twiddle_table(unsigned N, int direction, float *host_out, unsigned count) {
+    return guarded<int>(-1, [&] { return mufft_hip_twiddle_table_impl(N, direction, host_out, count); });
 }
 
 }  // extern "C"
@@ -1651,7 +1685,7 @@ int mufft_hip_set_variant(unsigned N, int kind, int variant) {
     return 0;
 }
 
-int mufft_hip_autotune_1d(unsigned N, int kind, unsigned howmany, double *best_us) {
+static int mufft_hip_autotune_1d_impl(unsigned N, int kind, unsigned howmany, double *best_us) {
     const int tk = tk_from_api(kind);
     const int l = tk < 0 ? -1 : logn_from_api(N, tk);
     if (l < 0 || howmany == 0) {
@@ -1760,8 +1794,13 @@ int mufft_hip_autotune_1d(unsigned N, int kind, unsigned howmany, double *best_u
     return best;
 }
 
+
+int mufft_hip_autotune_1d(unsigned N, int kind, unsigned howmany, double *best_us) {
+    return guarded<int>(-1, [&] { return mufft_hip_autotune_1d_impl(N, kind, howmany, best_us); });
+}
+
 // Tune cache: one line per entry, "log2n kind variant microseconds".
-int mufft_hip_tune_save(const char *path) {
+static int mufft_hip_tune_save_impl(const char *path) {
     FILE *f = fopen(path, "w");
     if (!f) {
         set_err("cannot write %s", path);
@@ -1781,7 +1820,12 @@ int mufft_hip_tune_save(const char *path) {
     return 0;
 }
 
-int mufft_hip_tune_load(const char *path) {
+
+int mufft_hip_tune_save(const char *path) {
+    return guarded<int>(-1, [&] { return mufft_hip_tune_save_impl(path); });
+}
+
+static int mufft_hip_tune_load_impl(const char *path) {
     FILE *f = fopen(path, "r");
     if (!f) {
         set_err("cannot read %s", path);
@@ -1800,6 +1844,11 @@ int mufft_hip_tune_load(const char *path) {
     }
     fclose(f);
     return n;
+}
+
+
+int mufft_hip_tune_load(const char *path) {
+    return guarded<int>(-1, [&] { return mufft_hip_tune_load_impl(path); });
 }
 
 }  // extern "C"
@@ -1845,7 +1894,7 @@ extern "C" {
 
 void mufft_hip_free_conv_batch(mufft_hip_conv_batch *b) { destroy_batch(b); }
 
-mufft_hip_conv_batch *mufft_hip_create_conv_batch(const mufft_hip_conv_job *jobs, unsigned count,
+static mufft_hip_conv_batch *mufft_hip_create_conv_batch_impl(const mufft_hip_conv_job *jobs, unsigned count,
                                                   unsigned flags) {
     (void)flags;  // CPU ISA bits have no meaning here
     if (!jobs || count == 0) {
@@ -2053,6 +2102,12 @@ mufft_hip_conv_batch *mufft_hip_create_conv_batch(const mufft_hip_conv_job *jobs
     return b;
 }
 
+
+mufft_hip_conv_batch *mufft_hip_create_conv_batch(const mufft_hip_conv_job *jobs, unsigned count,
+                                                  unsigned flags) {
+    return guarded<mufft_hip_conv_batch *>(nullptr, [&] { return mufft_hip_create_conv_batch_impl(jobs, count, flags); });
+}
+
 int mufft_hip_execute_conv_batch(mufft_hip_conv_batch *b, void *stream) {
     if (!b) {
         set_err("null batch");
@@ -2123,7 +2178,7 @@ void mufft_hip_free_ola(mufft_hip_ola *o) {
     delete o;
 }
 
-mufft_hip_ola *mufft_hip_create_ola(unsigned N, const float *taps, unsigned ntaps) {
+static mufft_hip_ola *mufft_hip_create_ola_impl(unsigned N, const float *taps, unsigned ntaps) {
     if (!is_pow2(N) || N < 4 || N > 65536 || !taps || ntaps == 0 || ntaps > N / 2) {
         set_err("invalid size or taps (N power of two in [4, 65536], 1 <= taps <= N/2)");
         return nullptr;
@@ -2162,6 +2217,11 @@ mufft_hip_ola *mufft_hip_create_ola(unsigned N, const float *taps, unsigned ntap
     }
     dh.release();
     return o;
+}
+
+
+mufft_hip_ola *mufft_hip_create_ola(unsigned N, const float *taps, unsigned ntaps) {
+    return guarded<mufft_hip_ola *>(nullptr, [&] { return mufft_hip_create_ola_impl(N, taps, ntaps); });
 }
 
 int mufft_hip_reset_ola(mufft_hip_ola *o, void *stream) {
