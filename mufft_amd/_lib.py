@@ -6,6 +6,7 @@ importing the package raises, so nothing can silently run on the CPU.
 from __future__ import annotations
 
 import ctypes
+import sys
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libmufft_hip.so"
@@ -82,6 +83,20 @@ class ConvJob(ctypes.Structure):
 _lib = None
 
 
+def _share_torch_runtime() -> None:
+    """PyTorch's ROCm wheels carry their own HIP and HSA runtimes.  Loaded
+    first, this library would bring /opt/rocm's copies into the process as
+    well, and whichever pair initialises second finds no device (torch then
+    reports no GPU).  Importing torch first, when it is installed, makes the
+    library bind to torch's runtime, so both share one."""
+    if "torch" in sys.modules:
+        return
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def load() -> ctypes.CDLL:
     global _lib
     if _lib is None:
@@ -89,6 +104,7 @@ def load() -> ctypes.CDLL:
             raise ImportError(
                 f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`"
                 " or `python mufft_amd/build.py`")
+        _share_torch_runtime()
         L = ctypes.CDLL(str(LIB_PATH))
         for name, res, args in SIGNATURES:
             fn = getattr(L, name)

@@ -9,6 +9,19 @@ import pytest
 from mufft_amd import _lib
 
 
+def test_loading_the_library_first_leaves_one_hip_runtime():
+    # The package binds to torch's bundled HIP runtime; a second copy from
+    # /opt/rocm in the same process would leave torch without a device.
+    import subprocess
+    import sys
+    code = ("import mufft_amd, torch\n"
+            "libs = {l.split()[-1] for l in open('/proc/self/maps') if 'libamdhip64' in l}\n"
+            "print(len(libs))\n")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert r.stdout.strip().splitlines()[-1] == "1"
+
+
 @pytest.fixture(scope="module")
 def lib():
     return _lib.load()
