@@ -33,12 +33,9 @@ __device__ __forceinline__ float2 mul_dir_i(float2 a) {
     else return make_float2(a.y, -a.x);
 }
 
-// a * exp(DIR * 2 pi i * J / R) with a compile-time exponent; the eighth-turn
-// cases use the exact structure of the constant instead of a full multiply.
-template <int J, int R, int DIR>
-__device__ __forceinline__ float2 twc(float2 a) {
-    static_assert(R <= 64 && (64 % R) == 0, "radix too large for the constant table");
-    constexpr int j = ((J % R) * (64 / R)) & 63;  // in 64ths of a turn
+// a * exp(DIR * 2 pi i * j / 64).
+template <int j, int DIR>
+__device__ __forceinline__ float2 twc64(float2 a) {
     constexpr float h = 0.70710678118654752440f;
     constexpr float d = DIR > 0 ? 1.0f : -1.0f;
     if constexpr (j == 0) return a;
@@ -53,6 +50,27 @@ __device__ __forceinline__ float2 twc(float2 a) {
         constexpr float c = (float)cos64(j);
         constexpr float s = (float)(d * sin64(j));
         return make_float2(a.x * c - a.y * s, a.x * s + a.y * c);
+    }
+}
+
+// a * exp(DIR * 2 pi i * J / R) with a compile-time exponent; the eighth-turn
+// cases use the exact structure of the constant instead of a full multiply.
+// R is a power of two; above 64 the constant comes from cos_turn / sin_turn.
+template <int J, int R, int DIR>
+__device__ __forceinline__ float2 twc(float2 a) {
+    static_assert(R > 0 && (R & (R - 1)) == 0, "R is a power of two");
+    if constexpr (R > 64) {
+        constexpr int jr = ((J % R) + R) % R;
+        if constexpr (jr % (R / 64) == 0) {
+            return twc<jr / (R / 64), 64, DIR>(a);
+        } else {
+            constexpr float d = DIR > 0 ? 1.0f : -1.0f;
+            constexpr float c = (float)cos_turn(jr, R);
+            constexpr float s = (float)(d * sin_turn(jr, R));
+            return make_float2(a.x * c - a.y * s, a.x * s + a.y * c);
+        }
+    } else {
+        return twc64<((((J % R) + R) % R) * (64 / R)) & 63, DIR>(a);
     }
 }
 

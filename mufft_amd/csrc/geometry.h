@@ -274,4 +274,35 @@ MUFFT_HD constexpr double sin64(int j) {
     return t[j & 63];
 }
 
+// cos and sin of 2*pi*j/r for any r > 0, in double, at compile time: the
+// quadrant is split off in integers and the remaining angle, in [0, pi/2),
+// goes through a Taylor series whose truncation error is below 1e-30.
+MUFFT_HD constexpr double taylor_sin(double x) {
+    double t = x, s = x;
+    for (int n = 1; n < 20; ++n) {
+        t *= -x * x / (double)((2 * n) * (2 * n + 1));
+        s += t;
+    }
+    return s;
+}
+MUFFT_HD constexpr double taylor_cos(double x) {
+    double t = 1.0, s = 1.0;
+    for (int n = 1; n < 20; ++n) {
+        t *= -x * x / (double)((2 * n - 1) * (2 * n));
+        s += t;
+    }
+    return s;
+}
+MUFFT_HD constexpr double turn_part(long long j, long long r, bool want_sin) {
+    const long long jj = ((j % r) + r) % r;
+    const long long q = (4 * jj) / r;            // quadrant
+    const long long rem = 4 * jj - q * r;        // angle within it, in units of (pi/2)/r
+    const double phi = 1.5707963267948966 * (double)rem / (double)r;
+    const double c = taylor_cos(phi), s = taylor_sin(phi);
+    if (want_sin) return q == 0 ? s : (q == 1 ? c : (q == 2 ? -s : -c));
+    return q == 0 ? c : (q == 1 ? -s : (q == 2 ? -c : s));
+}
+MUFFT_HD constexpr double cos_turn(long long j, long long r) { return turn_part(j, r, false); }
+MUFFT_HD constexpr double sin_turn(long long j, long long r) { return turn_part(j, r, true); }
+
 }  // namespace mufft

@@ -82,6 +82,13 @@ __device__ __forceinline__ void row_transform(const float2 *X, const float2 *Y, 
         }
     };
 
+    // Resolve twiddles W[k] = i*DIR*exp(i*pi*DIR*k/N) (the table's direction is
+    // the kernel's: r2c forward, c2r inverse).  A thread's points k = lt + T*c
+    // take W[lt] * exp(i*pi*DIR*c/E), a compile-time rotation, so one table
+    // entry per thread is read instead of one per point; W[N - k] = conj(W[k]).
+    float2 wb = make_float2(0.f, 0.f);
+    if constexpr (KIND != KIND_C2C) wb = ld_rtw(lt, 0);
+
     float2 v[E];
     if constexpr (KIND == KIND_C2R) {
         // Pre-pass resolve (kernel.c:36-45), optionally after the fused multiply.
@@ -110,7 +117,7 @@ __device__ __forceinline__ void row_transform(const float2 *X, const float2 *Y, 
                     xb = c == 0 && lt == 0 ? make_float2(xb.x, 0.f) : cscale(xb, 0.5f);
                 }
                 const float2 bc = cconj(xb);
-                z = cadd(cadd(xa, bc), cmul(csub(xa, bc), ld_rtw(lt, T * c)));
+                z = cadd(cadd(xa, bc), cmul(csub(xa, bc), twc<c, 2 * E, DIR>(wb)));
             }
             (void)k;
             v[c] = z;
@@ -151,11 +158,11 @@ __device__ __forceinline__ void row_transform(const float2 *X, const float2 *Y, 
         // 2*R_k for interior bins, R_0 - i R_N at bin 0 and R_N - i R_0 at bin N,
         // where R = r2c of the incoming gradient.
         const float scale = adj ? 1.0f : 0.5f;
-        // Bin (vp + sp) from za = Z[k] and zn = Z[N - k].
-        auto emit = [&](int vp, int sp, float2 za, float2 zn) {
+        // Bin (vp + sp) from za = Z[k], zn = Z[N - k] and w = W[k].
+        auto emit = [&](int vp, int sp, float2 za, float2 zn, float2 w) {
             const float2 zb = cconj(zn);
             const float2 fe = cadd(za, zb);
-            const float2 fo = cmul(ld_rtw(vp, sp), csub(za, zb));
+            const float2 fo = cmul(w, csub(za, zb));
             st_out(vp, sp, cscale(cadd(fe, fo), scale));
             if (full) st_out(vp, sp + N, cscale(csub(fe, fo), scale));
         };
@@ -175,12 +182,13 @@ __device__ __forceinline__ void row_transform(const float2 *X, const float2 *Y, 
                     st_out(0, 0, make_float2(r0, adj ? -rn : 0.f));
                     st_out(0, N, make_float2(rn, adj ? -r0 : 0.f));
                     const float2 zh = Lc[slot(0)];
-                    emit(0, N / 2, zh, zh);
+                    emit(0, N / 2, zh, zh, twc<E / 2, 2 * E, DIR>(wb));   // W[N/2], lt = 0
                 } else {
                     // Z[N - k] for k = lt + T*c is upper-half point N/2 - k.
                     const float2 zn = Lc[slot(N / 2 - lt - T * c)];
-                    emit(lt, T * c, v[c], zn);
-                    emit(T - lt, T * (E - 1 - c), zn, v[c]);
+                    const float2 w = twc<c, 2 * E, DIR>(wb);
+                    emit(lt, T * c, v[c], zn, w);
+                    emit(T - lt, T * (E - 1 - c), zn, v[c], cconj(w));
                 }
             });
         }

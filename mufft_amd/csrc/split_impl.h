@@ -70,6 +70,20 @@ __device__ __forceinline__ void split_transform(const float2 *X, const float2 *Y
         return make_float2(c, s);
     };
 
+    // Per-point twiddles from one table entry per thread and compile-time
+    // rotations (the tables' direction is the kernel's):
+    //   resolve  W[m] = i*DIR*exp(i*pi*DIR*m/NT), W[m + 2T*c] = W[m] * exp(i*pi*DIR*c/E),
+    //            W[NT - m] = conj(W[m]);
+    //   split    w_NT^(k1*n) for k1 = 1: w_NT^(lt + T*c) = w_NT^lt * exp(2*pi*i*DIR*c/(2E)).
+    // With N1 = 4 the split twiddle's rotation depends on k1, so it is read per point.
+    float2 wb = make_float2(0.f, 0.f);
+    if constexpr (KIND == KIND_C2R) wb = ld(rrtw, lt, 0);
+    if constexpr (KIND == KIND_R2C) wb = bload(rrtw, (unsigned)(2 * lt) * 8u, (unsigned)k1 * 8u);
+    float2 sb = make_float2(0.f, 0.f);
+    if constexpr (N1 == 2) {
+        if (k1 > 0) sb = ld(rstw, lt, 0);
+    }
+
     float2 v[E];
     static_for<0, E>([&](auto C) {
         constexpr int c = decltype(C)::value;
@@ -100,7 +114,8 @@ __device__ __forceinline__ void split_transform(const float2 *X, const float2 *Y
                     xb = m0 ? make_float2(xb.x, 0.f) : cscale(xb, 0.5f);
                 }
                 const float2 bc = cconj(xb);
-                z = cadd(cadd(xa, bc), cmul(csub(xa, bc), ld(rrtw, lt, sp)));
+                // W[lt + T*c + N2*n1] = W[lt] * exp(2*pi*i*DIR*(c + E*n1)/(4E)).
+                z = cadd(cadd(xa, bc), cmul(csub(xa, bc), twc<c + E * n1, 4 * E, DIR>(wb)));
             } else if (KIND == KIND_C2C && mul) {
                 float2 y = ld(rin2, lt, sp);
                 if (cj) y = cconj(y);
@@ -110,7 +125,11 @@ __device__ __forceinline__ void split_transform(const float2 *X, const float2 *Y
             }
             acc = n1 == 0 ? z : cadd(acc, cmul(z, rot(n1)));
         });
-        if (k1 > 0) acc = cmul(acc, ld(rstw, lt, (k1 - 1) * N2 + T * c));
+        if constexpr (N1 == 2) {
+            if (k1 > 0) acc = cmul(acc, twc<c, 2 * E, DIR>(sb));
+        } else {
+            if (k1 > 0) acc = cmul(acc, ld(rstw, lt, (k1 - 1) * N2 + T * c));
+        }
         v[c] = acc;
     });
 
@@ -130,13 +149,13 @@ __device__ __forceinline__ void split_transform(const float2 *X, const float2 *Y
         // upper half (k2 >= N2/2) goes through LDS once and the owner of each
         // lower-half point emits both bins of its pair.
         const float scale = adj ? 1.0f : 0.5f;
-        // Bin 2*(vp + sp) + k1 from za = Z[k] and zn = Z[NT - k].
-        auto emit = [&](int vp, int sp, float2 za, float2 zn) {
+        // Bin 2*(vp + sp) + k1 from za = Z[k], zn = Z[NT - k] and w = W[k].
+        auto emit = [&](int vp, int sp, float2 za, float2 zn, float2 w) {
             const unsigned vo = (unsigned)(2 * vp) * 8u;
             const unsigned so = (unsigned)(2 * sp + k1) * 8u;
             const float2 zb = cconj(zn);
             const float2 fe = cadd(za, zb);
-            const float2 fo = cmul(bload(rrtw, vo, so), csub(za, zb));
+            const float2 fo = cmul(w, csub(za, zb));
             bstore(rout, vo, so, cscale(cadd(fe, fo), scale));
             if (full) bstore(rout, vo, so + (unsigned)NT * 8u, cscale(csub(fe, fo), scale));
         };
@@ -155,12 +174,13 @@ __device__ __forceinline__ void split_transform(const float2 *X, const float2 *Y
                 bstore(rout, 0u, 0u, make_float2(r0, adj ? -rn : 0.f));
                 bstore(rout, 0u, (unsigned)NT * 8u, make_float2(rn, adj ? -r0 : 0.f));
                 const float2 zh = Lc[slot(0)];
-                emit(0, N2 / 2, zh, zh);
+                emit(0, N2 / 2, zh, zh, twc<E / 2, 2 * E, DIR>(wb));   // W[N2], lt = k1 = 0
             } else {
                 // Partner k2' = N2 - k2 - k1 = (T - lt - k1) + T*(E-1-c), upper half.
                 const float2 zn = Lc[slot(N2 / 2 - lt - T * c - k1)];
-                emit(lt, T * c, v[c], zn);
-                emit(T - lt - k1, T * (E - 1 - c), zn, v[c]);
+                const float2 w = twc<c, 2 * E, DIR>(wb);   // W[2*(lt + T*c) + k1]
+                emit(lt, T * c, v[c], zn, w);
+                emit(T - lt - k1, T * (E - 1 - c), zn, v[c], cconj(w));
             }
         });
     }
