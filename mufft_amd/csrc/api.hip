@@ -340,12 +340,13 @@ int build_split_table(int n, int n2, int direction, float2 **out, hipStream_t s)
     return 0;
 }
 
-// Radix sequence (R, ..., R, r_last) of the fused kernels with pass radix R.
-std::vector<int> fused_sequence(int n, int E) {
+// Radix sequence of the fused kernels with pass radix R: the remainder pass
+// first for the row kernels, last for the column kernels (seq_radix()).
+std::vector<int> fused_sequence(int n, int E, bool rem_first) {
     if (E > n) E = n;
     const int NP = E >= n ? 1 : (ilog2(n) + ilog2(E) - 1) / ilog2(E);
     std::vector<int> r;
-    for (int s = 0; s < NP; ++s) r.push_back(seq_radix(n, E, NP, s));
+    for (int s = 0; s < NP; ++s) r.push_back(seq_radix(n, E, NP, s, rem_first));
     return r;
 }
 
@@ -407,19 +408,19 @@ struct Engine {
         if (rows && row_fused()) {
             row_var = force_var ? force_var : chosen_variant(logn, tune_kind(kind, dir));
             const int n2 = rv_n(n, row_var);
-            if (build_stockham_table(n2, dir, fused_sequence(n2, rv_R(n, row_var)), &tw_row, s)) return -1;
+            if (build_stockham_table(n2, dir, fused_sequence(n2, rv_R(n, row_var), true), &tw_row, s)) return -1;
             if (rv_split(row_var) > 0 && build_split_table(n, n2, dir, &tw_split, s)) return -1;
         }
         if (cols && col_fused()) {
-            if (build_stockham_table(n, dir, fused_sequence(n, col_E(n)), &tw_col, s)) return -1;
+            if (build_stockham_table(n, dir, fused_sequence(n, col_E(n), false), &tw_col, s)) return -1;
         }
         if (rows && four_step()) {
             log_n1 = logn / 2;
             log_n2 = logn - log_n1;
             log_s = log_n2;
             const int n1 = 1 << log_n1, n2 = 1 << log_n2;
-            if (build_stockham_table(n1, dir, fused_sequence(n1, col_E(n1)), &tw_fs1, s) ||
-                build_stockham_table(n2, dir, fused_sequence(n2, col_E(n2)), &tw_fs2, s) ||
+            if (build_stockham_table(n1, dir, fused_sequence(n1, col_E(n1), false), &tw_fs1, s) ||
+                build_stockham_table(n2, dir, fused_sequence(n2, col_E(n2), false), &tw_fs2, s) ||
                 build_plain_table(n >> log_s, n >> log_s, dir, &tw_coarse, s) ||
                 build_plain_table(n, 1 << log_s, dir, &tw_fine, s))
                 return -1;
@@ -1533,7 +1534,7 @@ static int mufft_hip_twiddle_table_impl(unsigned N, int direction, float *host_o
     }
     hipStream_t s = nullptr;
     HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    std::vector<int> seq = fused_sequence((int)N, rv_R((int)N, rv_default((int)N)));
+    std::vector<int> seq = fused_sequence((int)N, rv_R((int)N, rv_default((int)N)), true);
     float2 *t = nullptr;
     int size = seq.size() > 1 ? (int)N - seq[0] : 0;
     int rc = build_stockham_table((int)N, direction, seq, &t, s);
@@ -1974,7 +1975,7 @@ static mufft_hip_conv_batch *mufft_hip_create_conv_batch_impl(const mufft_hip_co
         TabKey k{logn, kind, dir, nullptr, nullptr, nullptr};
         const int n = 1 << logn;
         const int n2 = logn >= CB_SPLIT_LOG2 ? n / 2 : n;
-        if (build_stockham_table(n2, dir, fused_sequence(n2, cb_points_per_thread(logn)), &k.tw, b->stream))
+        if (build_stockham_table(n2, dir, fused_sequence(n2, cb_points_per_thread(logn), true), &k.tw, b->stream))
             return -1;
         b->tables.push_back(k.tw);
         if (n2 != n) {

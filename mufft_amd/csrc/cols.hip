@@ -40,7 +40,7 @@ __global__ void __launch_bounds__(col_WG(L_)) fft_cols_kernel(ColArgs a) {
         v[c] = active ? src[(long long)(t + T * c) * a.in_rs] : make_float2(0.f, 0.f);
     });
 
-    stockham_passes<L_, E, DIR, XM, col_twm(L_)>(v, t, TwGlobal{a.tw}, smem, reinterpret_cast<float *>(smem),
+    stockham_passes<L_, E, DIR, XM, col_twm(L_), E, false, false>(v, t, TwGlobal{a.tw}, smem, reinterpret_cast<float *>(smem),
                                                  slot);
 
     if (active) {
@@ -85,7 +85,7 @@ __global__ void __launch_bounds__(fs_WG(L_)) fft_four_step_cols_kernel(FourStepA
         v[c] = (zp && row >= L_ / 2) ? make_float2(0.f, 0.f) : src[(long long)row * a.n2];
     });
 
-    stockham_passes<L_, E, DIR, XM_PAD, TW_FULL>(v, t, TwGlobal{a.tw}, smem, nullptr, slot);
+    stockham_passes<L_, E, DIR, XM_PAD, TW_FULL, E, false, false>(v, t, TwGlobal{a.tw}, smem, nullptr, slot);
 
     const int smask = (1 << a.log2s) - 1;
     static_for<0, E>([&](auto C) {

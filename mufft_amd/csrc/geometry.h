@@ -44,8 +44,9 @@ MUFFT_HD constexpr int ilog2(long long n) { return n <= 1 ? 0 : 1 + ilog2(n >> 1
 //              (split_impl.h); E, T, the pass count and the LDS size then
 //              refer to the N/N1-point sub-transform;
 //   bits 20-23 workgroups per CU to budget registers for (0: no request);
-//   bits 24-27 log2 of the radix of every pass but the last (0: radix E);
-//              a radix below E gives each thread E/radix butterflies per pass.
+//   bits 24-27 log2 of the radix of every pass but the remainder pass (0:
+//              radix E); a radix below E gives each thread E/radix
+//              butterflies per pass.
 // The planner's static choice is rv_default(); the autotuner may pick any
 // registered variant.
 // ---------------------------------------------------------------------------
@@ -83,22 +84,37 @@ MUFFT_HD constexpr int rv_lds_per(int N, int var) {
 // Static choice per size and transform kind (c2c, or r2c/c2r whose fused
 // transform has N = half the real length), from sweeps of all registered
 // variants on MI355X at 2^27-2^28 points per launch: split-product twiddles
-// pay off from 1024 points on; c2c prefers 32 points per thread at 1024 and
-// 16384 while the real transforms, whose resolve adds registers, prefer 16;
-// 32768 needs the split LDS exchange to fit one workgroup; 65536 runs on two
-// workgroups per transform.
+// pay off from 1024 points on; c2c prefers 32 points per thread at 1024, 8192
+// and 16384 while the real transforms, whose resolve adds registers, prefer
+// 16; 32768 needs the split LDS exchange to fit one workgroup; 65536 runs on
+// two workgroups per transform.
 MUFFT_HD constexpr int rv_default(int N, bool real = false) {
     return N <= 512 ? rv_make(16, 0, 0)
-         : N <= 16384 ? ((N == 1024 || N == 16384) && !real ? rv_make(32, 0, 1) : rv_make(16, 0, 1))
+         : N <= 16384 ? ((N == 1024 || N == 8192 || N == 16384) && !real ? rv_make(32, 0, 1) : rv_make(16, 0, 1))
          : N == 32768 ? rv_make(32, 1, 1)
          : rv_make_split(1, rv_make(32, 1, 1));
 }
 
-// Radix of pass s for a radix sequence that uses E for all but the last pass.
-MUFFT_HD constexpr int seq_radix(int N, int E, int NP, int s) {
-    return s < NP - 1 ? E : (N >> (ilog2(E) * (NP - 1)));
+// Radix sequence of NP passes: NP - 1 passes of radix R and one of the
+// remainder N / R^(NP-1), placed first (REM_FIRST) or last.  The twiddle
+// reads of a pass cover (r - 1) * p table entries for stride p, so with the
+// remainder first the row kernels keep the later passes' table regions small
+// enough for the L1 cache (8192 points: passes 8, 32, 32 read 20 twiddles
+// per thread from 20 KB instead of 32, 32, 8 reading 38 from 56 KB; 4-6%
+// faster on MI355X, 2048 and 4096 real 3-5%).  The column kernels, whose
+// lanes of one column thread share each twiddle, keep the remainder last:
+// the other order made 1024-point columns 10-13% slower.
+MUFFT_HD constexpr int seq_rem(int N, int R, int NP) { return N >> (ilog2(R) * (NP - 1)); }
+MUFFT_HD constexpr int seq_radix(int N, int R, int NP, int s, bool rem_first) {
+    return rem_first ? (s == 0 ? seq_rem(N, R, NP) : R) : (s < NP - 1 ? R : seq_rem(N, R, NP));
 }
-MUFFT_HD constexpr int seq_p(int E, int s) { return 1 << (ilog2(E) * s); }
+MUFFT_HD constexpr int seq_p(int N, int R, int NP, int s, bool rem_first) {
+    return rem_first ? (s == 0 ? 1 : seq_rem(N, R, NP) << (ilog2(R) * (s - 1))) : (1 << (ilog2(R) * s));
+}
+// Radix of the first pass (the twiddle table holds N minus it entries).
+MUFFT_HD constexpr int seq_first(int N, int R, int NP, bool rem_first) {
+    return seq_radix(N, R, NP, 0, rem_first);
+}
 
 // ---------------------------------------------------------------------------
 // Fused column (strided-axis) transform of length L over a tile of TX

@@ -143,7 +143,7 @@ __device__ __forceinline__ void row_transform(const float2 *X, const float2 *Y, 
     }
 
     if constexpr (BUF) {
-        const unsigned ntw = rv_NP(N, VAR) > 1 ? (unsigned)(N - RAD) : 0u;
+        const unsigned ntw = (unsigned)(N - seq_first(N, RAD, rv_NP(N, VAR), true));   // 0 for one pass
         stockham_passes<N, E, DIR, XM, TWM, RAD, WAVE>(v, lt, TwBuffer{make_rsrc(tw, ntw * 8u)}, L, Lf, slot);
     } else {
         stockham_passes<N, E, DIR, XM, TWM, RAD, WAVE>(v, lt, TwGlobal{tw}, L, Lf, slot);

@@ -134,7 +134,7 @@ __device__ __forceinline__ void split_transform(const float2 *X, const float2 *Y
     });
 
     constexpr int RAD = rv_R(N2, VAR);
-    stockham_passes<N2, E, DIR, XM, TWM, RAD>(v, lt, TwBuffer{make_rsrc(tw, (N2 - RAD) * 8u)}, L, Lf,
+    stockham_passes<N2, E, DIR, XM, TWM, RAD>(v, lt, TwBuffer{make_rsrc(tw, (N2 - seq_first(N2, RAD, np_for(N2, RAD), true)) * 8u)}, L, Lf,
                                               slot);
 
     // v[c] now holds X[N1*k2 + k1] for k2 = lt + T*c.

@@ -73,17 +73,12 @@ struct TwBuffer {
     }
 };
 
-// Number of passes when every pass but the last has radix E.
+// Number of passes of a sequence whose passes have radix E but one (the
+// first, radix N / E^(passes-1)).
 MUFFT_HD constexpr int np_for(int N, int E) {
     return E >= N ? 1 : (ilog2(N) + ilog2(E) - 1) / ilog2(E);
 }
 
-// v[c] holds point lt + T*c on entry (T = N/E) and holds output point lt + T*c
-// on exit.  Every pass but the last has radix R (R <= E, default E); each
-// thread runs J = E/r butterflies of a pass, butterfly j being b = lt + T*j
-// over its points v[j + m*J].  `twl` reads the twiddle table of the sequence
-// (R, ..., R, r_last).  `Slot(o)` maps a point index to its LDS slot; `L` is
-// the complex LDS view, `Lf` the float view used by the split exchange.
 // Orders the LDS exchange between the threads of one transform.  When they
 // all sit in one wavefront (WAVE), the wave's own LDS accesses are already
 // executed in order, so only the compiler must not move LDS accesses across
@@ -100,8 +95,15 @@ __device__ __forceinline__ void exchange_sync() {
     }
 }
 
-template <int N, int E, int DIR, int XM, int TWM, int R = E, bool WAVE = false, typename TwLoad,
-          typename Slot>
+// v[c] holds point lt + T*c on entry (T = N/E) and holds output point lt + T*c
+// on exit.  The passes follow seq_radix(): radix R (R <= E, default E) and
+// one remainder pass, first or last (REM_FIRST); each thread runs J = E/r
+// butterflies of a pass, butterfly j being b = lt + T*j over its points
+// v[j + m*J].  `twl` reads the twiddle table of that sequence.  `Slot(o)`
+// maps a point index to its LDS slot; `L` is the complex LDS view, `Lf` the
+// float view used by the split exchange.
+template <int N, int E, int DIR, int XM, int TWM, int R = E, bool WAVE = false, bool REM_FIRST = true,
+          typename TwLoad, typename Slot>
 __device__ __forceinline__ void stockham_passes(float2 (&v)[E], int lt, TwLoad twl,
                                                 float2 *L, float *Lf, Slot slot) {
     constexpr int T = N / E;
@@ -110,8 +112,8 @@ __device__ __forceinline__ void stockham_passes(float2 (&v)[E], int lt, TwLoad t
     static_assert(!WAVE || T <= 64, "wave-local exchange needs the transform within one wave");
     static_for<0, NP>([&](auto S) {
         constexpr int s = decltype(S)::value;
-        constexpr int r = seq_radix(N, R, NP, s);
-        constexpr int p = seq_p(R, s);
+        constexpr int r = seq_radix(N, R, NP, s, REM_FIRST);
+        constexpr int p = seq_p(N, R, NP, s, REM_FIRST);
         constexpr int J = E / r;   // butterflies per thread in this pass
         static_for<0, J>([&](auto JJ) {
             constexpr int j = decltype(JJ)::value;
@@ -119,7 +121,7 @@ __device__ __forceinline__ void stockham_passes(float2 (&v)[E], int lt, TwLoad t
             static_for<0, r>([&](auto M) { u[decltype(M)::value] = v[j + decltype(M)::value * J]; });
             if constexpr (s > 0) {
                 const int k = (lt + T * j) & (p - 1);
-                apply_twiddles<r, p, TWM>(u, k, tw_offset(R, p), twl);
+                apply_twiddles<r, p, TWM>(u, k, tw_offset(seq_first(N, R, NP, REM_FIRST), p), twl);
             }
             dft<r, DIR>(u);
             static_for<0, r>([&](auto Q) { v[j + decltype(Q)::value * J] = u[decltype(Q)::value]; });

@@ -15,6 +15,7 @@ pytestmark = pytest.mark.gpu
 
 
 def host_table(N, d, E):
+    # Radix sequence of the fused kernels: the remainder first, then E.
     E = min(E, N)
     radix = []
     left = N
@@ -22,6 +23,7 @@ def host_table(N, d, E):
         r = min(E, left)
         radix.append(r)
         left //= r
+    radix = radix[-1:] + radix[:-1]
     out = []
     p = 1
     for s, r in enumerate(radix):

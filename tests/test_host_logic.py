@@ -50,6 +50,7 @@ def test_defaults_follow_the_measured_policy(lib):
     # 16 points per thread for the real transforms at 1024 complex points, 32 for c2c;
     # 32768 points in one workgroup; 65536 split over two workgroups.
     assert lib.mufft_hip_default_variant(1024, 0) & 0xFF == 32
+    assert lib.mufft_hip_default_variant(8192, 0) & 0xFF == 32
     assert lib.mufft_hip_default_variant(2048, 2) & 0xFF == 16
     assert (lib.mufft_hip_default_variant(32768, 0) >> 16) & 0xF == 0
     assert (lib.mufft_hip_default_variant(65536, 0) >> 16) & 0xF == 1
