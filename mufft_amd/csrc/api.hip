@@ -415,7 +415,7 @@ struct Engine {
             if (build_stockham_table(n, dir, fused_sequence(n, col_E(n), false), &tw_col, s)) return -1;
         }
         if (rows && four_step()) {
-            log_n1 = logn / 2;
+            log_n1 = fs_log_n1(logn);
             log_n2 = logn - log_n1;
             log_s = log_n2;
             const int n1 = 1 << log_n1, n2 = 1 << log_n2;

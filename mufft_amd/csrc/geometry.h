@@ -140,6 +140,15 @@ MUFFT_HD constexpr int col_twm(int L) { return (L == 1024 || L == 2048) ? 1 : 0;
 MUFFT_HD constexpr int col_WG(int L) { return col_T(L) * col_TX(L); }
 MUFFT_HD constexpr int col_lds_rows(int L) { return L + L / col_E(L); }
 constexpr int COL_MAX_LOG2 = 13;   // fused column kernel covers L <= 8192
+// Four-step split N = N1 * N2 for 2^17 <= N <= 2^26: log2 N1, the length of
+// the first launch's columns (the second launch's are N2 = N / N1).  From a
+// sweep of every split on MI355X at 512 MiB per launch; against the balanced
+// split it saves 14% at 2^21, 21% at 2^22 and 23% at 2^25, where the chosen
+// N2 is a 1024- or 4096-point column that tiles well.
+MUFFT_HD constexpr int fs_log_n1(int logn) {
+    return logn == 17 ? 9 : logn == 18 ? 8 : logn == 21 ? 11 : logn == 22 ? 12 : logn == 23 ? 11
+         : logn == 25 ? 13 : logn / 2;
+}
 constexpr int ROW_MAX_LOG2 = 16;   // fused row kernels (one or two workgroups per transform) cover N <= 65536
 constexpr int ROW_WG_MAX_LOG2 = 15;  // one workgroup per transform up to 32768 points
 
