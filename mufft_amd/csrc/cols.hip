@@ -40,8 +40,9 @@ __global__ void __launch_bounds__(col_WG(L_)) fft_cols_kernel(ColArgs a) {
         v[c] = active ? src[(long long)(t + T * c) * a.in_rs] : make_float2(0.f, 0.f);
     });
 
-    stockham_passes<L_, E, DIR, XM, col_twm(L_), E, false, false>(v, t, TwGlobal{a.tw}, smem, reinterpret_cast<float *>(smem),
-                                                 slot);
+    // Remainder pass last (seq_radix): the lanes of one column thread share its twiddles.
+    stockham_passes<L_, E, DIR, XM, col_twm(L_), E, false, false>(v, t, TwGlobal{a.tw}, smem,
+                                                                  reinterpret_cast<float *>(smem), slot);
 
     if (active) {
         float2 *dst = a.out + (long long)img * a.out_bs + col;

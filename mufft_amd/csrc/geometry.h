@@ -130,13 +130,16 @@ MUFFT_HD constexpr int fs_TX(int L) {
     return col_T(L) <= 32 ? 256 / col_T(L) : (col_T(L) <= 128 ? 8 : (L >= 8192 ? 2 : 4));
 }
 MUFFT_HD constexpr int fs_WG(int L) { return col_T(L) * fs_TX(L); }
-// The plain column kernel, from sweeps on MI355X: at L = 1024 a 16-column
-// tile (one 128-byte line per row) with the split LDS exchange and split
-// twiddles is 15% faster than 8 columns; at L = 2048 a 4-column tile with the
-// split exchange (more, smaller workgroups per CU) is 12% faster than 8.
-MUFFT_HD constexpr int col_TX(int L) { return L == 1024 ? 16 : (L == 2048 ? 4 : fs_TX(L)); }
-MUFFT_HD constexpr int col_xm(int L) { return (L == 1024 || L == 2048) ? 1 : 0; }    // XMode
-MUFFT_HD constexpr int col_twm(int L) { return (L == 1024 || L == 2048) ? 1 : 0; }   // TwMode
+// The plain column kernel, from sweeps on MI355X.  From L = 1024 on it uses
+// the split LDS exchange and split twiddles (half the LDS of a tile).  At
+// L = 1024 a 16-column tile (one 128-byte line per row) is 15% faster than 8
+// columns; at L = 2048 a 4-column tile (more, smaller workgroups per CU) is
+// 12% faster than 8; at L = 4096 the split exchange lets two 4-column tiles
+// share a CU (2D 4096^2: -5%); at L = 8192 it allows 4 columns instead of 2
+// (2D 8192^2: -6%, four-step 2^26: -19%).
+MUFFT_HD constexpr int col_TX(int L) { return L == 1024 ? 16 : (L == 2048 ? 4 : (L == 8192 ? 4 : fs_TX(L))); }
+MUFFT_HD constexpr int col_xm(int L) { return L >= 1024 ? 1 : 0; }    // XMode
+MUFFT_HD constexpr int col_twm(int L) { return L >= 1024 ? 1 : 0; }   // TwMode
 MUFFT_HD constexpr int col_WG(int L) { return col_T(L) * col_TX(L); }
 MUFFT_HD constexpr int col_lds_rows(int L) { return L + L / col_E(L); }
 constexpr int COL_MAX_LOG2 = 13;   // fused column kernel covers L <= 8192
