@@ -395,6 +395,11 @@ struct Engine {
     // both halves and the two-level twiddle tables.
     int log_n1 = 0, log_n2 = 0, log_s = 0;
     float2 *tw_fs1 = nullptr, *tw_fs2 = nullptr, *tw_coarse = nullptr, *tw_fine = nullptr;
+    // Three launches (fs3_split): N2 = A * B column passes, the middle one with
+    // the twiddle w_N2^(b*k_a) from its own two-level tables.
+    int log_a = 0, log_b = 0, log_s2 = 0;
+    float2 *tw_pa = nullptr, *tw_pb = nullptr, *tw_coarse2 = nullptr, *tw_fine2 = nullptr;
+    bool three_pass() const { return log_a > 0; }
     std::vector<int> gen_radix;
     bool row_fused() const { return logn <= ROW_MAX_LOG2; }
     bool col_fused() const { return logn <= COL_MAX_LOG2; }
@@ -415,15 +420,28 @@ struct Engine {
             if (build_stockham_table(n, dir, fused_sequence(n, col_E(n), false), &tw_col, s)) return -1;
         }
         if (rows && four_step()) {
-            log_n1 = fs_log_n1(logn);
+            const int f3 = fs3_split(logn);
+            log_n1 = f3 ? (f3 >> 8) : fs_log_n1(logn);
             log_n2 = logn - log_n1;
             log_s = log_n2;
             const int n1 = 1 << log_n1, n2 = 1 << log_n2;
             if (build_stockham_table(n1, dir, fused_sequence(n1, col_E(n1), false), &tw_fs1, s) ||
-                build_stockham_table(n2, dir, fused_sequence(n2, col_E(n2), false), &tw_fs2, s) ||
                 build_plain_table(n >> log_s, n >> log_s, dir, &tw_coarse, s) ||
                 build_plain_table(n, 1 << log_s, dir, &tw_fine, s))
                 return -1;
+            if (f3) {
+                log_a = (f3 >> 4) & 15;
+                log_b = f3 & 15;
+                log_s2 = log_n2 / 2;
+                const int na = 1 << log_a, nb = 1 << log_b;
+                if (build_stockham_table(na, dir, fused_sequence(na, col_E(na), false), &tw_pa, s) ||
+                    build_stockham_table(nb, dir, fused_sequence(nb, col_E(nb), false), &tw_pb, s) ||
+                    build_plain_table(n2 >> log_s2, n2 >> log_s2, dir, &tw_coarse2, s) ||
+                    build_plain_table(n2, 1 << log_s2, dir, &tw_fine2, s))
+                    return -1;
+            } else if (build_stockham_table(n2, dir, fused_sequence(n2, col_E(n2), false), &tw_fs2, s)) {
+                return -1;
+            }
         }
         if ((rows && !row_fused() && !four_step()) || (cols && !col_fused())) {
             gen_radix = generic_sequence(n);
@@ -436,10 +454,11 @@ struct Engine {
         if (tw_col) (void)hipFree(tw_col);
         if (tw_gen) (void)hipFree(tw_gen);
         if (tw_split) (void)hipFree(tw_split);
-        for (float2 *t : {tw_fs1, tw_fs2, tw_coarse, tw_fine})
+        for (float2 *t : {tw_fs1, tw_fs2, tw_coarse, tw_fine, tw_pa, tw_pb, tw_coarse2, tw_fine2})
             if (t) (void)hipFree(t);
         tw_row = tw_col = tw_gen = tw_split = nullptr;
         tw_fs1 = tw_fs2 = tw_coarse = tw_fine = nullptr;
+        tw_pa = tw_pb = tw_coarse2 = tw_fine2 = nullptr;
     }
     int launches_generic() const { return (int)gen_radix.size(); }
 };
@@ -484,9 +503,9 @@ int run_generic(const Engine &e, const float2 *src, Lay s_lay, Lay fin, Lay alt,
 }
 
 // c2c transforms beyond the fused row sizes, packed scratch `alt` of
-// batch * n points: four-step (two fused column launches) up to
-// 2^(2 * COL_MAX_LOG2) points, generic passes beyond.  src, dst and alt must
-// be distinct.
+// batch * n points: four-step (two fused column launches, three for the
+// sizes of fs3_split) up to 2^(2 * COL_MAX_LOG2) points, generic passes
+// beyond.  src, dst and alt must be distinct.
 int run_large(const Engine &e, const float2 *src, long long sd, float2 *dst, long long dd,
               float2 *alt, int batch, bool zp, hipStream_t st) {
     if (!e.four_step())
@@ -507,6 +526,44 @@ int run_large(const Engine &e, const float2 *src, long long sd, float2 *dst, lon
     f.log2s = e.log_s;
     f.flags = zp ? RF_ZERO_PAD : 0u;
     HIP_TRY(launch_four_step_cols(e.log_n1, e.direction, f, st));
+    if (e.three_pass()) {
+        // alt holds y[n2][k1], n2 = B*a + b.  Middle pass, in place: for each
+        // (b, k1) the A-point transform over a (rows of B*N1 points), output
+        // k_a times w_N2^(b*k_a).
+        const int na = 1 << e.log_a, nb = 1 << e.log_b;
+        ColArgs c{};
+        c.in = alt;
+        c.out = alt;
+        c.tw = e.tw_pa;
+        c.in_rs = c.out_rs = (long long)nb * n1;
+        c.in_bs = c.out_bs = e.n;
+        c.ncols = nb * n1;
+        c.ntiles = (c.ncols + col_TX(na) - 1) / col_TX(na);
+        c.batch = batch;
+        c.coarse = e.tw_coarse2;
+        c.fine = e.tw_fine2;
+        c.log2s = e.log_s2;
+        c.log2div = e.log_n1;
+        HIP_TRY(launch_cols(e.log_a, e.direction, c, st, COL_TWIDDLE));
+        // Last pass: for each (k_a, k1) the B-point transform over b; output
+        // k_b is X[k1 + N1*(k_a + A*k_b)].  Image g = (transform, k_a).
+        ColArgs d{};
+        d.in = alt;
+        d.out = dst;
+        d.tw = e.tw_pb;
+        d.in_rs = n1;
+        d.out_rs = (long long)na * n1;
+        d.in_bs = (long long)nb * n1;
+        d.out_bs = n1;
+        d.img_div = na;
+        d.in_bs2 = e.n;
+        d.out_bs2 = dd;
+        d.ncols = n1;
+        d.ntiles = (n1 + col_TX(nb) - 1) / col_TX(nb);
+        d.batch = batch * na;
+        HIP_TRY(launch_cols(e.log_b, e.direction, d, st, COL_TWO_LEVEL));
+        return 0;
+    }
     // Second half: the N2-point column transforms of the n2 x n1 matrix.
     ColArgs c{};
     c.in = alt;
@@ -1049,7 +1106,7 @@ int mufft_hip_stream_copy(void *dst, const void *src, size_t bytes, void *stream
 int mufft_hip_plan_1d_launches(const mufft_plan_1d *p) {
     if (!p) return -1;
     if (p->eng.row_fused()) return 1;
-    const int c2c = p->eng.four_step() ? 2 : p->eng.launches_generic();
+    const int c2c = p->eng.four_step() ? (p->eng.three_pass() ? 3 : 2) : p->eng.launches_generic();
     return c2c + (p->kind == KIND_C2C ? 0 : 1);
 }
 
@@ -1265,7 +1322,7 @@ int mufft_hip_execute_plan_2d_batched(mufft_plan_2d *p, void *output, const void
 int mufft_hip_plan_2d_launches(const mufft_plan_2d *p) {
     if (!p) return -1;
     int rows = p->ex.row_fused() ? 1
-             : (p->ex.four_step() ? 2 : p->ex.launches_generic()) + (p->kind == KIND_C2C ? 0 : 1);
+             : (p->ex.four_step() ? (p->ex.three_pass() ? 3 : 2) : p->ex.launches_generic()) + (p->kind == KIND_C2C ? 0 : 1);
     int cols = p->ey.col_fused() ? 1 : p->ey.launches_generic();
     return rows + cols;
 }

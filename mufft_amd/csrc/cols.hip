@@ -13,7 +13,7 @@
 
 namespace mufft {
 
-template <int L_, int DIR>
+template <int L_, int DIR, int MODE = COL_PLAIN>
 __global__ void __launch_bounds__(col_WG(L_)) fft_cols_kernel(ColArgs a) {
     constexpr int E = col_E(L_);
     constexpr int LOGE = ilog2(E);
@@ -33,8 +33,14 @@ __global__ void __launch_bounds__(col_WG(L_)) fft_cols_kernel(ColArgs a) {
     const bool active = col < a.ncols && img < a.batch;
     auto slot = [x](int o) { return pad_slot<LOGE>(o) * TX + x; };
 
+    long long ib = (long long)img * a.in_bs, ob = (long long)img * a.out_bs;
+    if constexpr (MODE == COL_TWO_LEVEL) {
+        ib = (long long)(img / a.img_div) * a.in_bs2 + (long long)(img % a.img_div) * a.in_bs;
+        ob = (long long)(img / a.img_div) * a.out_bs2 + (long long)(img % a.img_div) * a.out_bs;
+    }
+
     float2 v[E];
-    const float2 *src = a.in + (long long)img * a.in_bs + col;
+    const float2 *src = a.in + ib + col;
     static_for<0, E>([&](auto C) {
         constexpr int c = decltype(C)::value;
         v[c] = active ? src[(long long)(t + T * c) * a.in_rs] : make_float2(0.f, 0.f);
@@ -45,7 +51,16 @@ __global__ void __launch_bounds__(col_WG(L_)) fft_cols_kernel(ColArgs a) {
                                                                   reinterpret_cast<float *>(smem), slot);
 
     if (active) {
-        float2 *dst = a.out + (long long)img * a.out_bs + col;
+        if constexpr (MODE == COL_TWIDDLE) {
+            const unsigned b = (unsigned)col >> a.log2div;
+            const unsigned smask = (1u << a.log2s) - 1u;
+            static_for<0, E>([&](auto C) {
+                constexpr int c = decltype(C)::value;
+                const unsigned e = b * (unsigned)(t + T * c);
+                v[c] = cmul(v[c], cmul(a.coarse[e >> a.log2s], a.fine[e & smask]));
+            });
+        }
+        float2 *dst = a.out + ob + col;
         static_for<0, E>([&](auto C) {
             constexpr int c = decltype(C)::value;
             dst[(long long)(t + T * c) * a.out_rs] = v[c];
@@ -152,9 +167,30 @@ static const void *col_kernel_ptr(int logL) {
     }
 }
 
-hipError_t launch_cols(int logL, int direction, const ColArgs &a, hipStream_t s) {
-    const void *k = direction < 0 ? col_kernel_ptr<-1>(logL) : col_kernel_ptr<1>(logL);
+// The middle and last passes of the three-pass transform: columns of 64 to
+// 1024 points.
+template <int DIR, int MODE>
+static const void *col_mode_kernel_ptr(int logL) {
+    switch (logL) {
+    case 6: return (const void *)&fft_cols_kernel<64, DIR, MODE>;
+    case 7: return (const void *)&fft_cols_kernel<128, DIR, MODE>;
+    case 8: return (const void *)&fft_cols_kernel<256, DIR, MODE>;
+    case 9: return (const void *)&fft_cols_kernel<512, DIR, MODE>;
+    case 10: return (const void *)&fft_cols_kernel<1024, DIR, MODE>;
+    default: return nullptr;
+    }
+}
+
+hipError_t launch_cols(int logL, int direction, const ColArgs &a, hipStream_t s, int mode) {
+    const void *k = nullptr;
+    if (mode == COL_PLAIN) k = direction < 0 ? col_kernel_ptr<-1>(logL) : col_kernel_ptr<1>(logL);
+    else if (mode == COL_TWIDDLE)
+        k = direction < 0 ? col_mode_kernel_ptr<-1, COL_TWIDDLE>(logL) : col_mode_kernel_ptr<1, COL_TWIDDLE>(logL);
+    else if (mode == COL_TWO_LEVEL)
+        k = direction < 0 ? col_mode_kernel_ptr<-1, COL_TWO_LEVEL>(logL) : col_mode_kernel_ptr<1, COL_TWO_LEVEL>(logL);
     if (!k) return hipErrorInvalidValue;
+    if (mode == COL_TWIDDLE && (!a.coarse || !a.fine)) return hipErrorInvalidValue;
+    if (mode == COL_TWO_LEVEL && a.img_div <= 0) return hipErrorInvalidValue;
     const int L = 1 << logL;
     const long long nblocks = (long long)a.ntiles * a.batch;
     if (nblocks <= 0) return hipSuccess;

@@ -148,6 +148,19 @@ constexpr int COL_MAX_LOG2 = 13;   // fused column kernel covers L <= 8192
 // sweep of every split on MI355X at 512 MiB per launch; against the balanced
 // split it saves 14% at 2^21, 21% at 2^22 and 23% at 2^25, where the chosen
 // N2 is a 1024- or 4096-point column that tiles well.
+// Three launches for the largest sizes, N = N1 * A * B: the four-step column
+// kernel on N1-point columns, then A-point and B-point column passes (at most
+// 1024 points, tiles of 64-128-byte row segments) over N2 = A * B.  Returns
+// log2 N1 << 8 | log2 A << 4 | log2 B, or 0 where two launches are used.
+// From a sweep of the splits on MI355X at 512 MiB per launch: against the
+// best two-launch split, -8% at 2^24, -16% at 2^25, -25% at 2^26 (2^23 and
+// below: no gain).
+MUFFT_HD constexpr int fs3_split(int logn) {
+    return logn == 24 ? (10 << 8) | (8 << 4) | 6
+         : logn == 25 ? (9 << 8) | (8 << 4) | 8
+         : logn == 26 ? (10 << 8) | (9 << 4) | 7
+         : 0;
+}
 MUFFT_HD constexpr int fs_log_n1(int logn) {
     return logn == 17 ? 9 : logn == 18 ? 8 : logn == 21 ? 11 : logn == 22 ? 12 : logn == 23 ? 11
          : logn == 25 ? 13 : logn / 2;

@@ -33,7 +33,19 @@ struct ColArgs {
     int ncols;                 // columns to transform (<= row stride)
     int ntiles;                // column tiles per image
     int batch;                 // images
+    // Mode COL_TWIDDLE: output row r of column c is multiplied by
+    // w_M^(dir*(c >> log2div)*r), w_M^e = coarse[e >> log2s] * fine[e & (2^log2s - 1)].
+    const float2 *coarse, *fine;
+    int log2s, log2div;
+    // Mode COL_TWO_LEVEL: image g sits at (g / img_div) * in_bs2 + (g % img_div) * in_bs
+    // (and likewise for the output).
+    int img_div;
+    long long in_bs2, out_bs2;
 };
+
+// Column-kernel modes (the plain one serves the 2D and four-step passes; the
+// other two are the middle and last passes of the three-pass transform).
+enum ColMode : int { COL_PLAIN = 0, COL_TWIDDLE = 1, COL_TWO_LEVEL = 2 };
 
 // First half of a four-step transform of N = N1 * N2 points (n = n1*N2 + n2):
 // the N1-point transforms down the columns of the N1 x N2 matrix, times
@@ -112,7 +124,7 @@ hipError_t launch_rows(int logN, int direction, int kind, int var, const RowArgs
 // Registered variants of a size; returns the count (at most max are written).
 int row_variant_list(int logN, int *vars, int max);
 bool row_variant_registered(int logN, int var);
-hipError_t launch_cols(int logL, int direction, const ColArgs &a, hipStream_t s);
+hipError_t launch_cols(int logL, int direction, const ColArgs &a, hipStream_t s, int mode = COL_PLAIN);
 hipError_t launch_pass(int radix, int direction, const PassArgs &a, hipStream_t s);
 hipError_t launch_resolve_r2c(const ResolveArgs &a, hipStream_t s);
 hipError_t launch_resolve_c2r(const ResolveArgs &a, hipStream_t s);

@@ -1,5 +1,5 @@
-"""Transforms beyond the fused sizes: four-step (two fused column launches)
-for 2^17..2^26 points, and generic passes beyond that.
+"""Transforms beyond the fused sizes: four-step (two fused column launches,
+three from 2^24 on) for 2^17..2^26 points, and generic passes beyond that.
 
 Every split N = N1 * N2 used by the planner is checked against an FP64 FFT
 (numpy's pocketfft in double, itself checked against the direct DFT in
@@ -14,8 +14,14 @@ from tests.helpers import abs_tol, cplx, empty_dev, from_dev, real, rel_rms, to_
 
 pytestmark = pytest.mark.gpu
 
+THREE_PASS_FROM = 24   # first size (log2) the planner splits into three launches
 
-@pytest.mark.parametrize("logn", list(range(17, 25)) + [26])
+
+def c2c_launches(logn):
+    return 3 if logn >= THREE_PASS_FROM else 2
+
+
+@pytest.mark.parametrize("logn", list(range(17, 27)))
 @pytest.mark.parametrize("direction", [-1, 1])
 def test_four_step_c2c(gpu, logn, direction):
     import torch
@@ -23,7 +29,7 @@ def test_four_step_c2c(gpu, logn, direction):
     N = 1 << logn
     batch, pad = 2, 24
     p = M.Plan1D("c2c", N, direction)
-    assert p.launches == 2
+    assert p.launches == c2c_launches(logn)
     x = cplx(0x9000 + logn, batch * (N + pad), 0.5).reshape(batch, N + pad)
     dy = torch.full((batch, N + pad), 3.0 + 0j, dtype=torch.complex64, device=gpu)
     p.execute_batched(dy, to_dev(x, gpu), batch, N + pad, N + pad)
@@ -38,7 +44,7 @@ def test_four_step_c2c(gpu, logn, direction):
             assert np.max(np.abs(y[b, :N] - ref)) < abs_tol(N), (logn, direction, b)
 
 
-@pytest.mark.parametrize("logn", [17, 20])
+@pytest.mark.parametrize("logn", [17, 20, 24])
 def test_four_step_zero_pad(gpu, logn):
     import torch
     import mufft_amd as M
@@ -53,7 +59,7 @@ def test_four_step_zero_pad(gpu, logn):
     assert rel_rms(from_dev(dy), np.fft.fft(xf)) < 2e-6
 
 
-@pytest.mark.parametrize("logn", [18, 21, 24])
+@pytest.mark.parametrize("logn", [18, 21, 24, 25])
 def test_four_step_real_roundtrip(gpu, logn):
     import torch
     import mufft_amd as M
@@ -61,7 +67,8 @@ def test_four_step_real_roundtrip(gpu, logn):
     x = real(0x9200 + logn, 2 * N, 0.5).reshape(2, N)
     p = M.Plan1D("r2c", N)
     q = M.Plan1D("c2r", N)
-    assert p.launches == 3 and q.launches == 3
+    # an N-point real transform is an N/2-point complex one plus the resolve pass
+    assert p.launches == q.launches == c2c_launches(logn - 1) + 1
     dX = empty_dev((2, N // 2 + 1), np.complex64, gpu)
     p.execute_batched(dX, to_dev(x, gpu), 2)
     dy = empty_dev((2, N), np.float32, gpu)
