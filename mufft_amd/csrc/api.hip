@@ -379,7 +379,7 @@ int tune_kind(int kind, int dir) {
 int chosen_variant(int logn, int tk) {
     std::lock_guard<std::mutex> g(g_var_lock);
     const int v = g_variant[logn][tk];
-    return v ? v : rv_default(1 << logn, tk >= 2);
+    return v ? v : rv_default(1 << logn, tk);
 }
 
 // A complex transform of length n along one axis: fused row kernel, fused
@@ -1718,7 +1718,7 @@ int mufft_hip_default_variant(unsigned N, int kind) {
         set_err("no fused kernel for this size");
         return -1;
     }
-    return rv_default(1 << l, tk >= 2);
+    return rv_default(1 << l, tk);
 }
 
 int mufft_hip_get_variant(unsigned N, int kind) {

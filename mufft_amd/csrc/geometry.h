@@ -81,17 +81,23 @@ MUFFT_HD constexpr int rv_NP(int N, int var) {
 MUFFT_HD constexpr int rv_lds_per(int N, int var) {
     return rv_xm(var) == 2 ? rv_n(N, var) : rv_n(N, var) + rv_n(N, var) / rv_E(N, var);
 }
-// Static choice per size and transform kind (c2c, or r2c/c2r whose fused
-// transform has N = half the real length), from sweeps of all registered
-// variants on MI355X at 2^27-2^28 points per launch: split-product twiddles
-// pay off from 1024 points on; c2c prefers 32 points per thread at 1024, 8192
-// and 16384 while the real transforms, whose resolve adds registers, prefer
-// 16; 32768 needs the split LDS exchange to fit one workgroup; 65536 runs on
-// two workgroups per transform.
-MUFFT_HD constexpr int rv_default(int N, bool real = false) {
+// Static choice per size and transform kind (0/1 c2c, 2 r2c, 3 c2r; the real
+// transforms' fused size N is half the real length), from sweeps of all
+// registered variants on MI355X at 2^27-2^28 points per launch: split-product
+// twiddles pay off from 1024 points on; c2c prefers 32 points per thread at
+// 1024, 8192 and 16384 while the real transforms, whose resolve adds
+// registers, prefer 16 up to 8192.  At 16384 r2c runs 32 points per thread
+// with registers budgeted for two workgroups per CU (-10%) and c2r 32 points
+// with the padded exchange; at 32768 c2c needs the split exchange to fit one
+// workgroup, r2c splits each transform over two workgroups of 16 points per
+// thread (-17%) and c2r over two workgroups budgeted to share a CU (-5%);
+// 65536 runs on two workgroups per transform.
+MUFFT_HD constexpr int rv_default(int N, int kind = 0) {
     return N <= 512 ? rv_make(16, 0, 0)
-         : N <= 16384 ? ((N == 1024 || N == 8192 || N == 16384) && !real ? rv_make(32, 0, 1) : rv_make(16, 0, 1))
-         : N == 32768 ? rv_make(32, 1, 1)
+         : N < 16384 ? ((N == 1024 || N == 8192) && kind < 2 ? rv_make(32, 0, 1) : rv_make(16, 0, 1))
+         : N == 16384 ? (kind == 2 ? rv_with_occ(rv_make(32, 1, 1), 2) : rv_make(32, 0, 1))
+         : N == 32768 ? (kind == 2 ? rv_make_split(1, rv_make(16, 0, 1))
+                       : kind == 3 ? rv_make_split(1, rv_with_occ(rv_make(32, 1, 1), 2)) : rv_make(32, 1, 1))
          : rv_make_split(1, rv_make(32, 1, 1));
 }
 

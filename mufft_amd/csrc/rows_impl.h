@@ -86,8 +86,9 @@ __device__ __forceinline__ void row_transform(const float2 *X, const float2 *Y, 
     // the kernel's: r2c forward, c2r inverse).  A thread's points k = lt + T*c
     // take W[lt] * exp(i*pi*DIR*c/E), a compile-time rotation, so one table
     // entry per thread is read instead of one per point; W[N - k] = conj(W[k]).
+    // (r2c reads W[lt] after the passes, where it is used.)
     float2 wb = make_float2(0.f, 0.f);
-    if constexpr (KIND != KIND_C2C) wb = ld_rtw(lt, 0);
+    if constexpr (KIND == KIND_C2R) wb = ld_rtw(lt, 0);
 
     float2 v[E];
     if constexpr (KIND == KIND_C2R) {
@@ -158,6 +159,7 @@ __device__ __forceinline__ void row_transform(const float2 *X, const float2 *Y, 
         // 2*R_k for interior bins, R_0 - i R_N at bin 0 and R_N - i R_0 at bin N,
         // where R = r2c of the incoming gradient.
         const float scale = adj ? 1.0f : 0.5f;
+        wb = ld_rtw(lt, 0);   // W[lt]
         // Bin (vp + sp) from za = Z[k], zn = Z[N - k] and w = W[k].
         auto emit = [&](int vp, int sp, float2 za, float2 zn, float2 w) {
             const float2 zb = cconj(zn);

@@ -76,9 +76,9 @@ __device__ __forceinline__ void split_transform(const float2 *X, const float2 *Y
     //            W[NT - m] = conj(W[m]);
     //   split    w_NT^(k1*n) for k1 = 1: w_NT^(lt + T*c) = w_NT^lt * exp(2*pi*i*DIR*c/(2E)).
     // With N1 = 4 the split twiddle's rotation depends on k1, so it is read per point.
+    // (r2c reads its base W[2*lt + k1] after the passes, where it is used.)
     float2 wb = make_float2(0.f, 0.f);
     if constexpr (KIND == KIND_C2R) wb = ld(rrtw, lt, 0);
-    if constexpr (KIND == KIND_R2C) wb = bload(rrtw, (unsigned)(2 * lt) * 8u, (unsigned)k1 * 8u);
     float2 sb = make_float2(0.f, 0.f);
     if constexpr (N1 == 2) {
         if (k1 > 0) sb = ld(rstw, lt, 0);
@@ -149,6 +149,7 @@ __device__ __forceinline__ void split_transform(const float2 *X, const float2 *Y
         // upper half (k2 >= N2/2) goes through LDS once and the owner of each
         // lower-half point emits both bins of its pair.
         const float scale = adj ? 1.0f : 0.5f;
+        wb = bload(rrtw, (unsigned)(2 * lt) * 8u, (unsigned)k1 * 8u);   // W[2*lt + k1]
         // Bin 2*(vp + sp) + k1 from za = Z[k], zn = Z[NT - k] and w = W[k].
         auto emit = [&](int vp, int sp, float2 za, float2 zn, float2 w) {
             const unsigned vo = (unsigned)(2 * vp) * 8u;
