@@ -74,7 +74,7 @@ struct TwBuffer {
 };
 
 // Number of passes of a sequence whose passes have radix E but one (the
-// first, radix N / E^(passes-1)).
+// remainder, radix N / E^(passes-1); see seq_radix()).
 MUFFT_HD constexpr int np_for(int N, int E) {
     return E >= N ? 1 : (ilog2(N) + ilog2(E) - 1) / ilog2(E);
 }
