@@ -60,6 +60,8 @@ def lib():
             ("mufft_get_convolve_func", vp, [u]),
             ("mufft_alloc", vp, [sz]),
             ("mufft_free", None, [vp]),
+            # oracle/ref_bench.c: the CPU-baseline workload on native threads
+            ("mufft_ref_bench_run", ctypes.c_double, [i, i, u, vp, i, u, vp, i]),
         ]:
             fn = getattr(L, name)
             fn.restype = res

@@ -284,3 +284,16 @@ def test_splitmix64_known_answer():
     u = inputs.uniform(5, 100000)
     assert u.min() >= -1.0 and u.max() < 1.0 and abs(float(u.mean())) < 0.02
     np.testing.assert_array_equal(inputs.uniform(5, 10, offset=3), u[3:13])
+
+
+def test_reference_baseline_driver(ref_lib):
+    """oracle/ref_bench.c runs the benchmark workload on native threads: it
+    reports a positive time for a valid run and -1 for invalid arguments or
+    sizes the reference refuses (plan creation returns NULL)."""
+    L = ref_lib.lib()
+    xc = np.ascontiguousarray(inputs.complex_uniform(0x2001, 8 * 4096), dtype=np.complex64)
+    xr = np.ascontiguousarray(inputs.uniform(0x2002, 2 * 65536), dtype=np.float32)
+    t = L.mufft_ref_bench_run(4, 2, 4096, xc.ctypes.data, 8, 65536, xr.ctypes.data, 2)
+    assert t > 0
+    assert L.mufft_ref_bench_run(0, 1, 4096, xc.ctypes.data, 8, 65536, xr.ctypes.data, 2) == -1.0
+    assert L.mufft_ref_bench_run(2, 1, 1000, xc.ctypes.data, 1, 65536, xr.ctypes.data, 1) == -1.0
