@@ -86,18 +86,18 @@ MUFFT_HD constexpr int rv_lds_per(int N, int var) {
 // registered variants on MI355X at 2^27-2^28 points per launch: split-product
 // twiddles pay off from 1024 points on; c2c prefers 32 points per thread at
 // 1024, 8192 and 16384 while the real transforms, whose resolve adds
-// registers, prefer 16 up to 8192.  At 16384 r2c runs 32 points per thread
-// with registers budgeted for two workgroups per CU (-10%) and c2r 32 points
-// with the padded exchange; at 32768 c2c needs the split exchange to fit one
-// workgroup, r2c splits each transform over two workgroups of 16 points per
-// thread (-17%) and c2r over two workgroups budgeted to share a CU (-5%);
+// registers, prefer 16 up to 8192.  At 16384 c2c and r2c run 32 points per
+// thread with the split exchange and registers budgeted for two workgroups
+// per CU (c2c -18%, r2c -10%), c2r 32 points with the padded exchange; at
+// 32768 r2c splits each transform over two workgroups of 16 points per thread
+// (-17%), c2c and c2r over two workgroups budgeted to share a CU (-11%, -5%);
 // 65536 runs on two workgroups per transform.
 MUFFT_HD constexpr int rv_default(int N, int kind = 0) {
     return N <= 512 ? rv_make(16, 0, 0)
          : N < 16384 ? ((N == 1024 || N == 8192) && kind < 2 ? rv_make(32, 0, 1) : rv_make(16, 0, 1))
-         : N == 16384 ? (kind == 2 ? rv_with_occ(rv_make(32, 1, 1), 2) : rv_make(32, 0, 1))
+         : N == 16384 ? (kind == 3 ? rv_make(32, 0, 1) : rv_with_occ(rv_make(32, 1, 1), 2))
          : N == 32768 ? (kind == 2 ? rv_make_split(1, rv_make(16, 0, 1))
-                       : kind == 3 ? rv_make_split(1, rv_with_occ(rv_make(32, 1, 1), 2)) : rv_make(32, 1, 1))
+                                   : rv_make_split(1, rv_with_occ(rv_make(32, 1, 1), 2)))
          : rv_make_split(1, rv_make(32, 1, 1));
 }
 

@@ -48,11 +48,13 @@ def test_every_size_has_variants_and_a_registered_default(lib, kind):
 
 def test_defaults_follow_the_measured_policy(lib):
     # 16 points per thread for the real transforms at 1024 complex points, 32 for c2c;
-    # c2c 32768 in one workgroup; 65536 split over two workgroups.
+    # c2c 16384 with registers for two workgroups per CU; 32768 and 65536 split
+    # over two workgroups.
     assert lib.mufft_hip_default_variant(1024, 0) & 0xFF == 32
     assert lib.mufft_hip_default_variant(8192, 0) & 0xFF == 32
     assert lib.mufft_hip_default_variant(2048, 2) & 0xFF == 16
-    assert (lib.mufft_hip_default_variant(32768, 0) >> 16) & 0xF == 0
+    assert (lib.mufft_hip_default_variant(16384, 0) >> 20) & 0xF == 2
+    assert (lib.mufft_hip_default_variant(32768, 0) >> 16) & 0xF == 1
     assert (lib.mufft_hip_default_variant(65536, 0) >> 16) & 0xF == 1
     # the real transforms of 65536 samples (32768 complex points) split in two
     assert (lib.mufft_hip_default_variant(65536, 2) >> 16) & 0xF == 1
