@@ -103,7 +103,9 @@ int mufft_hip_get_variant(unsigned N, int kind);
 /* variant 0 restores the static default. */
 int mufft_hip_set_variant(unsigned N, int kind, int variant);
 /* Times every variant on howmany transforms, checks each against an FP64 FFT
- * (relative RMS <= 3e-6) and selects the fastest that passes; returns it. */
+ * (relative RMS <= 3e-6) and selects the fastest that passes; returns it.
+ * Each timed launch starts with cold caches (a 256 MiB stream evicts the
+ * Infinity Cache first), as a batch streamed from HBM would meet them. */
 int mufft_hip_autotune_1d(unsigned N, int kind, unsigned howmany, double *best_us);
 /* Tune cache as text, one "log2n kind variant us" line per tuned size. */
 int mufft_hip_tune_save(const char *path);

@@ -1769,12 +1769,23 @@ static int mufft_hip_autotune_1d_impl(unsigned N, int kind, unsigned howmany, do
 
     hipStream_t s = nullptr;
     HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    DevBuf din, dout, s0, s1;
+    DevBuf din, dout;
     float2 *rtw = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int best = -1;
     double best_t = 1e30;
     int rc = 0;
+    // Each timed launch starts with cold caches: streaming 256 MiB through
+    // this buffer evicts the 256 MB Infinity Cache and the L2s, as a batch
+    // streamed from HBM meets them.  Repeating a launch on the same buffers
+    // would instead find its input cached and favour the variants that read
+    // it twice (the two-workgroup split), which lose on cold data.
+    constexpr long long kFlushPts = 1LL << 25;   // 256 MiB of float2 per half
+    float2 *flush = nullptr;
+    if (hipMalloc(&flush, 2 * kFlushPts * sizeof(float2)) != hipSuccess) {
+        (void)hipGetLastError();   // no flush: warm timing
+        flush = nullptr;
+    }
     if (din.ensure(h_in.size() * sizeof(float2)) || dout.ensure((size_t)odist * howmany * sizeof(float2)) ||
         hipMemcpy(din.p, h_in.data(), h_in.size() * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess ||
         (fkind != KIND_C2C && build_resolve_table(nc, dir, &rtw, s)) ||
@@ -1817,13 +1828,18 @@ static int mufft_hip_autotune_1d_impl(unsigned N, int kind, unsigned howmany, do
         ok = ok && err <= 3e-6;
         if (ok) {
             const int reps = 5;
-            (void)launch_rows(l, dir, fkind, vars[i], a, s);
-            (void)hipEventRecord(e0, s);
-            for (int r = 0; r < reps; ++r) (void)launch_rows(l, dir, fkind, vars[i], a, s);
-            (void)hipEventRecord(e1, s);
-            float ms = 0;
-            ok = hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
-            const double us = 1e3 * ms / reps;
+            (void)launch_rows(l, dir, fkind, vars[i], a, s);   // warm-up
+            double total_ms = 0;
+            for (int r = 0; r < reps && ok; ++r) {
+                if (flush) (void)launch_stream_copy(flush, flush + kFlushPts, kFlushPts, s);
+                (void)hipEventRecord(e0, s);
+                (void)launch_rows(l, dir, fkind, vars[i], a, s);
+                (void)hipEventRecord(e1, s);
+                float ms = 0;
+                ok = hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+                total_ms += ms;
+            }
+            const double us = 1e3 * total_ms / reps;
             if (ok && us < best_t) {
                 best_t = us;
                 best = vars[i];
@@ -1833,6 +1849,9 @@ static int mufft_hip_autotune_1d_impl(unsigned N, int kind, unsigned howmany, do
         e.release();
     }
     if (rtw) (void)hipFree(rtw);
+    if (flush) (void)hipFree(flush);
+    din.release();
+    dout.release();
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     (void)hipStreamDestroy(s);

@@ -83,21 +83,23 @@ MUFFT_HD constexpr int rv_lds_per(int N, int var) {
 }
 // Static choice per size and transform kind (0/1 c2c, 2 r2c, 3 c2r; the real
 // transforms' fused size N is half the real length), from sweeps of all
-// registered variants on MI355X at 2^27-2^28 points per launch: split-product
-// twiddles pay off from 1024 points on; c2c prefers 32 points per thread at
-// 1024, 8192 and 16384 while the real transforms, whose resolve adds
-// registers, prefer 16 up to 8192.  At 16384 c2c and r2c run 32 points per
-// thread with the split exchange and registers budgeted for two workgroups
-// per CU (c2c -18%, r2c -10%), c2r 32 points with the padded exchange; at
-// 32768 r2c splits each transform over two workgroups of 16 points per thread
-// (-17%), c2c and c2r over two workgroups budgeted to share a CU (-11%, -5%);
-// 65536 runs on two workgroups per transform.
+// registered variants on MI355X at 2^27 points per launch (1 GiB buffers, so
+// inputs come from HBM, not from the 256 MB Infinity Cache): split-product
+// twiddles pay off from 512 points on; c2r below 1024 points runs 8 points
+// per thread (-10% to -27%); c2c prefers 32 points per thread at 1024 and
+// 8192 while the real transforms, whose resolve adds registers, prefer 16 up
+// to 8192.  At 16384 c2c and r2c run 32 points per thread with the split
+// exchange and registers budgeted for two workgroups per CU (-20%, -7%), c2r
+// 32 points with the padded exchange.  At 32768 one workgroup per transform
+// (c2c and c2r 32 points, r2c 64 points per thread) beats the two-workgroup
+// split by 20-34% on cold data; the split only wins while a batch stays in
+// the Infinity Cache.  65536 runs on two workgroups per transform.
 MUFFT_HD constexpr int rv_default(int N, int kind = 0) {
-    return N <= 512 ? rv_make(16, 0, 0)
+    return N < 512 ? (kind == 3 && N == 256 ? rv_make(8, 0, 0) : rv_make(16, 0, 0))
+         : N == 512 ? (kind == 3 ? rv_make(8, 0, 0) : rv_make(16, 0, 1))
          : N < 16384 ? ((N == 1024 || N == 8192) && kind < 2 ? rv_make(32, 0, 1) : rv_make(16, 0, 1))
          : N == 16384 ? (kind == 3 ? rv_make(32, 0, 1) : rv_with_occ(rv_make(32, 1, 1), 2))
-         : N == 32768 ? (kind == 2 ? rv_make_split(1, rv_make(16, 0, 1))
-                                   : rv_make_split(1, rv_with_occ(rv_make(32, 1, 1), 2)))
+         : N == 32768 ? (kind == 2 ? rv_make(64, 1, 1) : rv_make(32, 1, 1))
          : rv_make_split(1, rv_make(32, 1, 1));
 }
 

@@ -48,17 +48,17 @@ def test_every_size_has_variants_and_a_registered_default(lib, kind):
 
 def test_defaults_follow_the_measured_policy(lib):
     # 16 points per thread for the real transforms at 1024 complex points, 32 for c2c;
-    # c2c 16384 with registers for two workgroups per CU; 32768 and 65536 split
+    # c2r 8 points below 1024; c2c 16384 with registers for two workgroups per CU;
+    # 32768 complex points in one workgroup (r2c 64 points per thread); 65536 split
     # over two workgroups.
     assert lib.mufft_hip_default_variant(1024, 0) & 0xFF == 32
     assert lib.mufft_hip_default_variant(8192, 0) & 0xFF == 32
     assert lib.mufft_hip_default_variant(2048, 2) & 0xFF == 16
+    assert lib.mufft_hip_default_variant(1024, 3) & 0xFF == 8
     assert (lib.mufft_hip_default_variant(16384, 0) >> 20) & 0xF == 2
-    assert (lib.mufft_hip_default_variant(32768, 0) >> 16) & 0xF == 1
+    assert (lib.mufft_hip_default_variant(32768, 0) >> 16) & 0xF == 0
+    assert lib.mufft_hip_default_variant(65536, 2) & 0xFF == 64
     assert (lib.mufft_hip_default_variant(65536, 0) >> 16) & 0xF == 1
-    # the real transforms of 65536 samples (32768 complex points) split in two
-    assert (lib.mufft_hip_default_variant(65536, 2) >> 16) & 0xF == 1
-    assert (lib.mufft_hip_default_variant(65536, 3) >> 16) & 0xF == 1
 
 
 def test_invalid_sizes_and_variants_are_refused(lib):
