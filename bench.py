@@ -414,8 +414,8 @@ def main():
     # The real legs run right after two 1 GiB c2c launches, which per-transform
     # tuning does not see (on MI355X the tuned two-workgroup variants lost ~9%
     # of the step to the one-workgroup defaults there).  So the (r2c, c2r) pair
-    # is chosen by timing the whole step for the tuned variant and the static
-    # default of each transform, still at plan time.
+    # is chosen by timing the whole step for a few candidates of each
+    # transform, still at plan time.
     def pair_step_ms(rv, cv, reps=8):
         L.mufft_hip_set_variant(REAL_N, 2, rv)
         L.mufft_hip_set_variant(REAL_N, 3, cv)
@@ -439,8 +439,11 @@ def main():
         pc.close()
         return ms
 
-    cand_r = sorted({int(tuned[f"r2c_{REAL_N}"]["variant"], 16), L.mufft_hip_default_variant(REAL_N, 2)})
-    cand_c = sorted({int(tuned[f"c2r_{REAL_N}"]["variant"], 16), L.mufft_hip_default_variant(REAL_N, 3)})
+    # candidates: the tuned variant, the static default and one workgroup of
+    # 32 points per thread (V32ST), which won the step on some boxes
+    one_wg = 0x1120
+    cand_r = sorted({int(tuned[f"r2c_{REAL_N}"]["variant"], 16), L.mufft_hip_default_variant(REAL_N, 2), one_wg})
+    cand_c = sorted({int(tuned[f"c2r_{REAL_N}"]["variant"], 16), L.mufft_hip_default_variant(REAL_N, 3), one_wg})
     pair_ms = {(rv, cv): pair_step_ms(rv, cv) for rv in cand_r for cv in cand_c}
     best_r, best_c = min(pair_ms, key=pair_ms.get)
     L.mufft_hip_set_variant(REAL_N, 2, best_r)
