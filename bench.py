@@ -452,6 +452,10 @@ def main():
                  "step_ms": {f"{r:#x}/{c:#x}": round(v, 4) for (r, c), v in pair_ms.items()}}
     r2c = M.Plan1D("r2c", REAL_N)
     c2r = M.Plan1D("c2r", REAL_N)
+    # the pair applies to these plans only; later legs (other batches) get the
+    # per-transform tuned variants back
+    L.mufft_hip_set_variant(REAL_N, 2, int(tuned[f"r2c_{REAL_N}"]["variant"], 16))
+    L.mufft_hip_set_variant(REAL_N, 3, int(tuned[f"c2r_{REAL_N}"]["variant"], 16))
 
     ev = []
 
