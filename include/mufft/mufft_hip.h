@@ -82,6 +82,16 @@ int mufft_hip_execute_plan_1d_mul_batched(mufft_plan_1d *plan, void *output, con
 int mufft_hip_mul_reduce(void *output, const void *a, const void *b, unsigned n, unsigned howmany,
                          size_t adist, size_t bdist, float norm, int conj_b, void *stream);
 
+/* As mufft_create_plan_1d_c2c / _r2c / _c2r, with the fused-kernel variant
+ * fixed for this plan.  kind: 0 c2c forward, 1 c2c inverse, 2 r2c, 3 c2r (the
+ * codes of the variant calls below); N as for those creators; variant: a
+ * registered variant of the size (mufft_hip_list_variants), 0 = the
+ * process-wide choice.  No counterpart in the reference, whose plans always
+ * take its static decomposition (fft.c:325-379). */
+mufft_plan_1d *mufft_hip_create_plan_1d_variant(unsigned N, int kind, unsigned flags, int variant);
+/* The fused-kernel variant a plan runs (0 when it runs no fused row kernel). */
+int mufft_hip_plan_1d_variant(const mufft_plan_1d *plan);
+
 /* Number of kernel launches one execution of the plan issues. */
 int mufft_hip_plan_1d_launches(const mufft_plan_1d *plan);
 int mufft_hip_plan_2d_launches(const mufft_plan_2d *plan);
@@ -90,6 +100,17 @@ int mufft_hip_plan_2d_launches(const mufft_plan_2d *plan);
  * uses, generated in FP64 on the GPU; copied to host_out (count complex
  * values, at most the table size).  Returns the table size. */
 int mufft_hip_twiddle_table(unsigned N, int direction, float *host_out, unsigned count);
+
+/* The auxiliary FP64-built tables, for inspection (count complex values
+ * copied to host_out; returns the table size, or the size alone when
+ * host_out is NULL):
+ *   mode 1: the r2c / c2r resolve twiddles W[k] = (0, dir) * exp(i pi dir k / n),
+ *           k < n, of a transform of complex length n (fft.c:441-455);
+ *   mode 2: exp(dir * 2 pi i * j / n), j < param (four-step twiddles);
+ *   mode 3: exp(dir * 2 pi i * j * k1 / n) at (k1 - 1) * param + j, 0 < k1 < n / param,
+ *           j < param (twiddles of a transform split over n / param workgroups). */
+int mufft_hip_twiddle_table_mode(int mode, unsigned n, unsigned param, int direction,
+                                 float *host_out, unsigned count);
 
 /* ---- Fused-kernel variants and the autotuner ----
  * A size of up to 32768 complex points runs as one fused kernel whose

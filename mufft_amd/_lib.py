@@ -48,6 +48,9 @@ SIGNATURES = [
     ("mufft_hip_execute_plan_2d_batched", _i, [_vp, _vp, _vp, _u, _vp]),
     ("mufft_hip_execute_conv_input_batched", _i, [_vp, _u, _vp, _vp, _u, _sz, _sz, _vp]),
     ("mufft_hip_execute_conv_output_batched", _i, [_vp, _vp, _vp, _vp, _u, _sz, _sz, _sz, _vp]),
+    ("mufft_hip_create_plan_1d_variant", _vp, [_u, _i, _u, _i]),
+    ("mufft_hip_plan_1d_variant", _i, [_vp]),
+    ("mufft_hip_twiddle_table_mode", _i, [_i, _u, _u, _i, _vp, _u]),
     ("mufft_hip_plan_1d_launches", _i, [_vp]),
     ("mufft_hip_plan_2d_launches", _i, [_vp]),
     ("mufft_hip_twiddle_table", _i, [_u, _i, _vp, _u]),

@@ -49,6 +49,64 @@ def ptr(x) -> int | None:
     raise TypeError(f"unsupported buffer type {type(x)!r}")
 
 
+# Element types the kernels read and write: complex64 for complex data,
+# float32 for real data (an operand may also be given in the other of the two,
+# its size then counts in bytes).
+_REAL, _CPLX = 4, 8
+
+
+def check_buffer(x, need_bytes: int, what: str) -> None:
+    """Checks one operand before its address goes to the C ABI: contiguous,
+    float32 / complex64, on the GPU or in host memory, and at least
+    ``need_bytes`` long.  Raw integer addresses are taken as they are."""
+    if x is None or isinstance(x, int):
+        return
+    if hasattr(x, "data_ptr"):  # torch tensor
+        import torch
+        if x.dtype not in (torch.float32, torch.complex64):
+            raise ValueError(f"{what}: dtype {x.dtype} (float32 or complex64 expected)")
+        if not x.is_contiguous():
+            raise ValueError(f"{what}: tensor must be contiguous")
+        if x.device.type not in ("cuda", "cpu"):
+            raise ValueError(f"{what}: tensor on unsupported device {x.device}")
+        have = x.numel() * x.element_size()
+    elif isinstance(x, np.ndarray):
+        if x.dtype not in (np.float32, np.complex64):
+            raise ValueError(f"{what}: dtype {x.dtype} (float32 or complex64 expected)")
+        if not x.flags.c_contiguous:
+            raise ValueError(f"{what}: array must be C-contiguous")
+        have = x.nbytes
+    else:
+        raise TypeError(f"{what}: unsupported buffer type {type(x)!r}")
+    if have < need_bytes:
+        raise ValueError(f"{what}: {have} bytes, the call needs {need_bytes}")
+
+
+def span_bytes(howmany: int, dist: int, packed: int, elem: int) -> int:
+    """Bytes spanned by howmany blocks of `packed` elements at distance `dist`
+    (0 = packed) in elements of `elem` bytes."""
+    if howmany <= 0:
+        return 0
+    d = dist if dist else packed
+    return ((howmany - 1) * d + packed) * elem
+
+
+def layout_1d(kind: str, N: int, flags: int, adjoint: bool = False):
+    """(input elements, input element bytes, output elements, output element
+    bytes) of one transform of a 1D plan, or of its adjoint."""
+    zp = bool(flags & FLAG_ZERO_PAD_UPPER_HALF)
+    full = bool(flags & FLAG_FULL_R2C)
+    if kind == "c2c":
+        fwd = ((N // 2 if zp else N), _CPLX, N, _CPLX)
+        return (N, _CPLX, N, _CPLX) if adjoint else fwd
+    if kind == "r2c":
+        spec = N if full else N // 2 + 1
+        return (spec, _CPLX, N, _REAL) if adjoint else ((N // 2 if zp else N), _REAL, spec, _CPLX)
+    if kind == "c2r":
+        return (N, _REAL, N // 2 + 1, _CPLX) if adjoint else (N // 2 + 1, _CPLX, N, _REAL)
+    raise ValueError(kind)
+
+
 def stream_handle(s) -> int | None:
     if s is None:
         return None
@@ -68,11 +126,19 @@ def _check(rc: int, what: str):
 class Plan1D:
     """mufft_plan_1d (fft.h:78-121).  kind: 'c2c', 'r2c' or 'c2r'."""
 
-    def __init__(self, kind: str, N: int, direction: int = FORWARD, flags: int = 0):
+    def __init__(self, kind: str, N: int, direction: int = FORWARD, flags: int = 0,
+                 variant: int = 0):
+        """``variant`` fixes the fused-kernel variant of this plan (library
+        extension; 0 = the process-wide choice)."""
         L = _lib.load()
         self.kind, self.N, self.flags = kind, N, flags
         self.direction = direction if kind == "c2c" else (FORWARD if kind == "r2c" else INVERSE)
-        if kind == "c2c":
+        if variant:
+            code = {"c2c": 0 if direction == FORWARD else 1, "r2c": 2, "c2r": 3}.get(kind)
+            if code is None:
+                raise ValueError(kind)
+            h = L.mufft_hip_create_plan_1d_variant(N, code, flags, variant)
+        elif kind == "c2c":
             h = L.mufft_create_plan_1d_c2c(N, direction, flags)
         elif kind == "r2c":
             h = L.mufft_create_plan_1d_r2c(N, flags)
@@ -92,10 +158,26 @@ class Plan1D:
     def launches(self) -> int:
         return _lib.load().mufft_hip_plan_1d_launches(self._h)
 
+    @property
+    def variant(self) -> int:
+        """The fused-kernel variant this plan runs (0: no fused row kernel)."""
+        return int(_lib.load().mufft_hip_plan_1d_variant(self._h))
+
+    def _validate(self, out, inp, howmany, idist, odist, adjoint=False, inp2=None, idist2=0):
+        ni, ei, no, eo = layout_1d(self.kind, self.N, self.flags, adjoint)
+        if inp2 is not None and self.kind == "c2c":
+            ni = self.N  # the fused multiply reads every input point, zero padding or not
+        check_buffer(inp, span_bytes(howmany, idist, ni, ei), "input")
+        check_buffer(out, span_bytes(howmany, odist, no, eo), "output")
+        if inp2 is not None:
+            check_buffer(inp2, span_bytes(howmany if idist2 else 1, idist2, ni, ei), "second input")
+
     def execute(self, out, inp):
+        self._validate(out, inp, 1, 0, 0)
         _lib.load().mufft_execute_plan_1d(self._h, ptr(out), ptr(inp))
 
     def execute_batched(self, out, inp, howmany: int, idist: int = 0, odist: int = 0, stream=None):
+        self._validate(out, inp, howmany, idist, odist)
         rc = _lib.load().mufft_hip_execute_plan_1d_batched(
             self._h, ptr(out), ptr(inp), howmany, idist, odist, stream_handle(stream))
         _check(rc, "mufft_hip_execute_plan_1d_batched")
@@ -103,6 +185,7 @@ class Plan1D:
     def execute_adjoint_batched(self, out, inp, howmany: int, idist: int = 0, odist: int = 0,
                                 stream=None):
         """Adjoint (conjugate transpose) of the plan's transform; see mufft_hip.h."""
+        self._validate(out, inp, howmany, idist, odist, adjoint=True)
         rc = _lib.load().mufft_hip_execute_adjoint_1d_batched(
             self._h, ptr(out), ptr(inp), howmany, idist, odist, stream_handle(stream))
         _check(rc, "mufft_hip_execute_adjoint_1d_batched")
@@ -112,6 +195,7 @@ class Plan1D:
                             adjoint: bool = False, stream=None):
         """out = T(norm * inp * op(inp2)) with the multiply fused into the transform;
         idist2 = 0 applies one inp2 block to every transform."""
+        self._validate(out, inp, howmany, idist, odist, adjoint=adjoint, inp2=inp2, idist2=idist2)
         rc = _lib.load().mufft_hip_execute_plan_1d_mul_batched(
             self._h, ptr(out), ptr(inp), ptr(inp2), howmany, idist, idist2, odist,
             ctypes.c_float(norm), int(conj2), int(adjoint), stream_handle(stream))
@@ -151,10 +235,21 @@ class Plan2D:
     def launches(self) -> int:
         return _lib.load().mufft_hip_plan_2d_launches(self._h)
 
+    def _validate(self, out, inp, howmany):
+        px = self.Nx * self.Ny
+        # r2c output / c2r input rows hold Nx complex values, Nx/2+1 of them
+        # meaningful (fft.h:238-260); real images are Nx * Ny floats.
+        ei = _REAL if self.kind == "r2c" else _CPLX
+        eo = _REAL if self.kind == "c2r" else _CPLX
+        check_buffer(inp, howmany * px * ei, "input")
+        check_buffer(out, howmany * px * eo, "output")
+
     def execute(self, out, inp):
+        self._validate(out, inp, 1)
         _lib.load().mufft_execute_plan_2d(self._h, ptr(out), ptr(inp))
 
     def execute_batched(self, out, inp, howmany: int, stream=None):
+        self._validate(out, inp, howmany)
         rc = _lib.load().mufft_hip_execute_plan_2d_batched(
             self._h, ptr(out), ptr(inp), howmany, stream_handle(stream))
         _check(rc, "mufft_hip_execute_plan_2d_batched")

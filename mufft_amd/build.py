@@ -14,7 +14,14 @@ INCLUDE = ROOT / "include"
 BUILD = PKG / "_build"
 LIB = PKG / "libmufft_hip.so"
 
-SOURCES = ["rows_c2c_fwd.hip", "rows_c2c_inv.hip", "rows_r2c.hip", "rows_c2r.hip", "conv_batch.hip", "api.hip", "misc.hip", "cols.hip"]
+# Kernel units first (the row-kernel instantiations take the longest), then
+# the host units (planner, executor and C ABI).
+SOURCES = [
+    "rows_c2c_fwd.hip", "rows_c2c_inv.hip", "rows_r2c.hip", "rows_c2r.hip", "conv_batch.hip",
+    "cols.hip", "misc.hip",
+    "host_common.hip", "exec.hip", "plan_1d.hip", "plan_2d.hip", "plan_conv.hip", "runtime.hip",
+    "autotune.hip", "conv_batch_host.hip", "ola.hip",
+]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 # Packed-FP32 instructions are disabled for all device code.

@@ -182,147 +182,6 @@ constexpr int ROW_WG_MAX_LOG2 = 15;  // one workgroup per transform up to 32768 
 // has N - r_0 entries in total.
 MUFFT_HD constexpr int tw_offset(int r0, int p) { return p - r0; }
 
-// cos(2 pi j / 64) and sin(2 pi j / 64): constant twiddles of the in-register
-// radix-2..64 butterflies.
-MUFFT_HD constexpr double cos64(int j) {
-    constexpr double t[64] = {
-    1,
-    0.99518472667219693,
-    0.98078528040323043,
-    0.95694033573220882,
-    0.92387953251128674,
-    0.88192126434835505,
-    0.83146961230254524,
-    0.77301045336273699,
-    0.70710678118654757,
-    0.63439328416364549,
-    0.55557023301960229,
-    0.47139673682599781,
-    0.38268343236508984,
-    0.29028467725446233,
-    0.19509032201612833,
-    0.09801714032956077,
-    6.123233995736766e-17,
-    -0.098017140329560645,
-    -0.19509032201612819,
-    -0.29028467725446216,
-    -0.38268343236508973,
-    -0.4713967368259977,
-    -0.55557023301960196,
-    -0.63439328416364538,
-    -0.70710678118654746,
-    -0.77301045336273699,
-    -0.83146961230254535,
-    -0.88192126434835494,
-    -0.92387953251128674,
-    -0.95694033573220882,
-    -0.98078528040323043,
-    -0.99518472667219682,
-    -1,
-    -0.99518472667219693,
-    -0.98078528040323043,
-    -0.95694033573220894,
-    -0.92387953251128685,
-    -0.88192126434835505,
-    -0.83146961230254546,
-    -0.7730104533627371,
-    -0.70710678118654768,
-    -0.63439328416364593,
-    -0.55557023301960218,
-    -0.47139673682599786,
-    -0.38268343236509034,
-    -0.29028467725446244,
-    -0.19509032201612866,
-    -0.098017140329560451,
-    -1.8369701987210297e-16,
-    0.09801714032956009,
-    0.1950903220161283,
-    0.29028467725446205,
-    0.38268343236509,
-    0.47139673682599759,
-    0.55557023301960184,
-    0.6343932841636456,
-    0.70710678118654735,
-    0.77301045336273666,
-    0.83146961230254524,
-    0.88192126434835483,
-    0.92387953251128652,
-    0.95694033573220882,
-    0.98078528040323032,
-    0.99518472667219693
-    };
-    return t[j & 63];
-}
-MUFFT_HD constexpr double sin64(int j) {
-    constexpr double t[64] = {
-    0,
-    0.098017140329560604,
-    0.19509032201612825,
-    0.29028467725446233,
-    0.38268343236508978,
-    0.47139673682599764,
-    0.55557023301960218,
-    0.63439328416364549,
-    0.70710678118654746,
-    0.77301045336273699,
-    0.83146961230254524,
-    0.88192126434835494,
-    0.92387953251128674,
-    0.95694033573220894,
-    0.98078528040323043,
-    0.99518472667219682,
-    1,
-    0.99518472667219693,
-    0.98078528040323043,
-    0.95694033573220894,
-    0.92387953251128674,
-    0.88192126434835505,
-    0.83146961230254546,
-    0.7730104533627371,
-    0.70710678118654757,
-    0.63439328416364549,
-    0.55557023301960218,
-    0.47139673682599786,
-    0.38268343236508989,
-    0.29028467725446239,
-    0.19509032201612861,
-    0.098017140329560826,
-    1.2246467991473532e-16,
-    -0.09801714032956059,
-    -0.19509032201612836,
-    -0.29028467725446211,
-    -0.38268343236508967,
-    -0.47139673682599764,
-    -0.55557023301960196,
-    -0.63439328416364527,
-    -0.70710678118654746,
-    -0.77301045336273666,
-    -0.83146961230254524,
-    -0.88192126434835494,
-    -0.92387953251128652,
-    -0.95694033573220882,
-    -0.98078528040323032,
-    -0.99518472667219693,
-    -1,
-    -0.99518472667219693,
-    -0.98078528040323043,
-    -0.95694033573220894,
-    -0.92387953251128663,
-    -0.88192126434835505,
-    -0.83146961230254546,
-    -0.77301045336273688,
-    -0.70710678118654768,
-    -0.63439328416364593,
-    -0.55557023301960218,
-    -0.47139673682599792,
-    -0.38268343236509039,
-    -0.2902846772544625,
-    -0.19509032201612872,
-    -0.098017140329560506
-    };
-    return t[j & 63];
-}
-
 // cos and sin of 2*pi*j/r for any r > 0, in double, at compile time: the
 // quadrant is split off in integers and the remaining angle, in [0, pi/2),
 // goes through a Taylor series whose truncation error is below 1e-30.
@@ -353,5 +212,15 @@ MUFFT_HD constexpr double turn_part(long long j, long long r, bool want_sin) {
 }
 MUFFT_HD constexpr double cos_turn(long long j, long long r) { return turn_part(j, r, false); }
 MUFFT_HD constexpr double sin_turn(long long j, long long r) { return turn_part(j, r, true); }
+
+// cos(2 pi j / 64) and sin(2 pi j / 64): constant twiddles of the in-register
+// radix-2..64 butterflies (exact at the quarter turns).
+MUFFT_HD constexpr double cos64(int j) { return cos_turn(j, 64); }
+MUFFT_HD constexpr double sin64(int j) { return sin_turn(j, 64); }
+static_assert(cos64(16) == 0.0 && sin64(16) == 1.0 && cos64(32) == -1.0 && sin64(48) == -1.0,
+              "quarter turns are exact");
+static_assert(cos64(8) - 0.70710678118654752 < 4e-16 && 0.70710678118654752 - cos64(8) < 4e-16 &&
+                  sin64(4) - 0.38268343236508977 < 4e-16 && 0.38268343236508977 - sin64(4) < 4e-16,
+              "eighth and sixteenth turns within a few ulp");
 
 }  // namespace mufft

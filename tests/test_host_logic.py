@@ -108,3 +108,68 @@ def test_plan_creation_without_a_device_fails_cleanly(lib):
     assert not lib.mufft_create_plan_1d_c2c(1000, -1, 0)
     assert "size" in _lib.last_error()
     assert not lib.mufft_create_plan_1d_r2c(2, 0)
+
+
+# ---------------------------------------------------------------- argument checks
+def test_buffer_layouts_of_the_reference_plans():
+    from mufft_amd import api as A
+    # c2c N, zero-padded input N/2 (fft.h:63-65); r2c N reals -> N/2+1 bins (N with
+    # FULL_R2C); c2r N/2+1 bins -> N reals; the adjoint swaps the two sides.
+    assert A.layout_1d("c2c", 64, 0) == (64, 8, 64, 8)
+    assert A.layout_1d("c2c", 64, A.FLAG_ZERO_PAD_UPPER_HALF) == (32, 8, 64, 8)
+    assert A.layout_1d("r2c", 64, 0) == (64, 4, 33, 8)
+    assert A.layout_1d("r2c", 64, A.FLAG_FULL_R2C | A.FLAG_ZERO_PAD_UPPER_HALF) == (32, 4, 64, 8)
+    assert A.layout_1d("c2r", 64, 0) == (33, 8, 64, 4)
+    assert A.layout_1d("r2c", 64, 0, adjoint=True) == (33, 8, 64, 4)
+    assert A.layout_1d("c2r", 64, 0, adjoint=True) == (64, 4, 33, 8)
+    assert A.span_bytes(3, 0, 33, 8) == 3 * 33 * 8
+    assert A.span_bytes(3, 40, 33, 8) == (2 * 40 + 33) * 8
+    assert A.span_bytes(0, 40, 33, 8) == 0
+
+
+def _plan_stub(cls, **attrs):
+    # A plan object without a device handle: only its Python-side checks run.
+    p = cls.__new__(cls)
+    p.__dict__.update(attrs)
+    p._h = None
+    return p
+
+
+def test_wrappers_refuse_bad_buffers_before_the_c_call():
+    import numpy as np
+    import torch
+
+    from mufft_amd import api as A
+    p = _plan_stub(A.Plan1D, kind="c2c", N=64, flags=0, direction=-1)
+    good = np.zeros((4, 64), np.complex64)
+    p._validate(good, good, 4, 0, 0)                       # fits
+    with pytest.raises(ValueError, match="bytes"):
+        p._validate(np.zeros((3, 64), np.complex64), good, 4, 0, 0)   # output one transform short
+    with pytest.raises(ValueError, match="bytes"):
+        p._validate(good, good, 4, 0, 65)                  # distance makes the output span longer
+    with pytest.raises(ValueError, match="contiguous"):
+        p._validate(np.zeros((4, 128), np.complex64)[:, ::2], good, 4, 0, 0)
+    with pytest.raises(ValueError, match="dtype"):
+        p._validate(np.zeros((4, 64), np.complex128), good, 4, 0, 0)
+    t = torch.zeros((4, 128), dtype=torch.complex64)
+    with pytest.raises(ValueError, match="contiguous"):
+        p._validate(t[:, ::2], good, 4, 0, 0)
+    with pytest.raises(ValueError, match="dtype"):
+        p._validate(torch.zeros((4, 64), dtype=torch.float64), good, 4, 0, 0)
+    p._validate(torch.zeros((4, 64), dtype=torch.complex64), good, 4, 0, 0)
+    # a float32 buffer of the same byte size is accepted for complex data
+    p._validate(torch.zeros((4, 128), dtype=torch.float32), good, 4, 0, 0)
+    # the fused multiply's second operand: one block (idist2 = 0) or one per transform
+    p._validate(good, good, 4, 0, 0, inp2=np.zeros(64, np.complex64), idist2=0)
+    with pytest.raises(ValueError, match="second input"):
+        p._validate(good, good, 4, 0, 0, inp2=np.zeros(64, np.complex64), idist2=64)
+    r = _plan_stub(A.Plan1D, kind="r2c", N=64, flags=0, direction=-1)
+    r._validate(np.zeros((2, 33), np.complex64), np.zeros((2, 64), np.float32), 2, 0, 0)
+    with pytest.raises(ValueError, match="output"):
+        r._validate(np.zeros((2, 32), np.complex64), np.zeros((2, 64), np.float32), 2, 0, 0)
+    q = _plan_stub(A.Plan2D, kind="c2r", Nx=16, Ny=8)
+    q._validate(np.zeros(16 * 8, np.float32), np.zeros((8, 16), np.complex64), 1)
+    with pytest.raises(ValueError, match="input"):
+        q._validate(np.zeros(16 * 8, np.float32), np.zeros((8, 9), np.complex64), 1)
+    # raw addresses are passed through unchecked
+    p._validate(0x1000, 0x2000, 4, 0, 0)

@@ -62,6 +62,59 @@ def test_fp64_direct_convolution_matches_fft_convolution():
     assert abs(fp64.conv_direct(a, b)[5] - sum(a[5 - i] * b[i] for i in range(6))) < 1e-12
 
 
+def _c2r_brute(X, N):
+    """c2r by direct summation in FP64 with the reference's resolve semantics
+    (kernel.c:36-45): the Hermitian part of bins 0..N/2 plus the imaginary
+    parts of bins 0 and N/2 entering as -Im(X0)*(-1)^n - Im(X_{N/2})."""
+    X = np.asarray(X, np.complex128)
+    n = np.arange(N)
+    y = np.full(X.shape[:-1] + (N,), 0.0)
+    y += X[..., :1].real + X[..., N // 2: N // 2 + 1].real * np.where(n % 2 == 0, 1.0, -1.0)
+    for k in range(1, N // 2):
+        y += 2.0 * np.real(X[..., k: k + 1] * np.exp(2j * np.pi * k * n / N))
+    y -= X[..., :1].imag * np.where(n % 2 == 0, 1.0, -1.0) + X[..., N // 2: N // 2 + 1].imag
+    return y
+
+
+@pytest.mark.parametrize("N", [4, 8, 32])
+def test_fp64_c2r_matches_direct_sum(N):
+    X = inputs.complex_uniform(0x77 + N, 3 * (N // 2 + 1)).reshape(3, N // 2 + 1)
+    np.testing.assert_allclose(fp64.c2r(X, N), _c2r_brute(X, N), atol=1e-12 * N)
+    np.testing.assert_allclose(fp64.r2c(inputs.uniform(N, N)), fp64.dft_direct(inputs.uniform(N, N))[: N // 2 + 1],
+                               atol=1e-12 * N)
+
+
+@pytest.mark.parametrize("Nx,Ny", [(4, 8), (8, 2), (16, 4)])
+@pytest.mark.parametrize("d", [-1, 1])
+def test_fp64_2d_matches_separable_direct_dft(Nx, Ny, d):
+    """The 2D oracle against direct sums along each axis (rows, then columns)."""
+    x = inputs.complex_uniform(Nx * 16 + Ny, 2 * Nx * Ny).reshape(2, Ny, Nx)
+    rows = fp64.dft_direct(x, d)
+    brute = np.swapaxes(fp64.dft_direct(np.swapaxes(rows, -1, -2), d), -1, -2)
+    np.testing.assert_allclose(fp64.c2c_2d(x, d), brute, atol=1e-11 * Nx * Ny)
+    # and against the definition X[ky, kx] = sum x[y, x] w^(d (kx x / Nx + ky y / Ny)) at one bin
+    ky, kx = 1, Nx - 1
+    yy, xx = np.meshgrid(np.arange(Ny), np.arange(Nx), indexing="ij")
+    direct = np.sum(x[0] * np.exp(d * 2j * np.pi * (kx * xx / Nx + ky * yy / Ny)))
+    assert abs(fp64.c2c_2d(x, d)[0, ky, kx] - direct) < 1e-11 * Nx * Ny
+
+
+@pytest.mark.parametrize("Nx,Ny", [(4, 8), (16, 2)])
+def test_fp64_real_2d_match_direct_sums(Nx, Ny):
+    x = inputs.uniform(Nx + 3 * Ny, 2 * Nx * Ny).reshape(2, Ny, Nx)
+    rows = fp64.dft_direct(x, -1)
+    full = np.swapaxes(fp64.dft_direct(np.swapaxes(rows, -1, -2), -1), -1, -2)
+    np.testing.assert_allclose(fp64.r2c_2d(x), full[..., : Nx // 2 + 1], atol=1e-11 * Nx * Ny)
+    np.testing.assert_allclose(fp64.r2c_2d(x, full=True), full, atol=1e-11 * Nx * Ny)
+    # c2r_2d: inverse column sums, then the per-row c2r with the reference's DC /
+    # Nyquist imaginary-part terms, on an arbitrary (non-Hermitian) spectrum
+    X = inputs.complex_uniform(Nx * Ny, 2 * Ny * (Nx // 2 + 1)).reshape(2, Ny, Nx // 2 + 1)
+    cols = np.swapaxes(fp64.dft_direct(np.swapaxes(X, -1, -2), 1), -1, -2)
+    np.testing.assert_allclose(fp64.c2r_2d(X, Nx), _c2r_brute(cols, Nx), atol=1e-11 * Nx * Ny)
+    # and the round trip on the spectrum of a real image: c2r(r2c(x)) = Nx*Ny*x
+    np.testing.assert_allclose(fp64.c2r_2d(fp64.r2c_2d(x), Nx), Nx * Ny * x, atol=1e-10 * Nx * Ny)
+
+
 # --------------------------------------------------------------------------- golden
 @pytest.fixture(scope="module")
 def gold():
