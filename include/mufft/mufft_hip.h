@@ -42,6 +42,23 @@ int mufft_hip_synchronize(void *stream);
 int mufft_hip_execute_plan_1d_batched(mufft_plan_1d *plan, void *output, const void *input,
                                       unsigned howmany, size_t idist, size_t odist, void *stream);
 
+/* Allocation-free form of mufft_hip_execute_plan_1d_batched, with no
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, layouts and results (the same kernels run);
+ * input and output must be device memory or pinned host memory (pageable
+ * memory is refused).  Sizes that run as one fused kernel (up to 65536
+ * complex points) need no scratch.  Larger sizes take their scratch from a
+ * fixed 64 MiB device array, one per GPU, held until the process ends: the
+ * call is refused when N x howmany exceeds 2^22 complex points (N = the
+ * plan's complex length, N/2 of the real length for r2c / c2r).  All calls
+ * on one GPU share that array, so such calls that overlap in time (other
+ * streams, other threads, other plans) race and must be ordered by the
+ * caller.  The plan is created beforehand, as usual. */
+int mufft_hip_execute_plan_1d_batched_noalloc(mufft_plan_1d *plan, void *output, const void *input,
+                                              unsigned howmany, size_t idist, size_t odist,
+                                              void *stream);
+
 /* howmany packed 2D images (layouts of mufft_execute_plan_2d, one after another). */
 int mufft_hip_execute_plan_2d_batched(mufft_plan_2d *plan, void *output, const void *input,
                                       unsigned howmany, void *stream);

@@ -45,6 +45,7 @@ SIGNATURES = [
     ("mufft_hip_memcpy", _i, [_vp, _vp, _sz]),
     ("mufft_hip_synchronize", _i, [_vp]),
     ("mufft_hip_execute_plan_1d_batched", _i, [_vp, _vp, _vp, _u, _sz, _sz, _vp]),
+    ("mufft_hip_execute_plan_1d_batched_noalloc", _i, [_vp, _vp, _vp, _u, _sz, _sz, _vp]),
     ("mufft_hip_execute_plan_2d_batched", _i, [_vp, _vp, _vp, _u, _vp]),
     ("mufft_hip_execute_conv_input_batched", _i, [_vp, _u, _vp, _vp, _u, _sz, _sz, _vp]),
     ("mufft_hip_execute_conv_output_batched", _i, [_vp, _vp, _vp, _vp, _u, _sz, _sz, _sz, _vp]),

@@ -20,7 +20,7 @@ SOURCES = [
     "rows_c2c_fwd.hip", "rows_c2c_inv.hip", "rows_r2c.hip", "rows_c2r.hip", "conv_batch.hip",
     "cols.hip", "misc.hip",
     "host_common.hip", "exec.hip", "plan_1d.hip", "plan_2d.hip", "plan_conv.hip", "runtime.hip",
-    "autotune.hip", "conv_batch_host.hip", "ola.hip",
+    "autotune.hip", "conv_batch_host.hip", "ola.hip", "noalloc.hip",
 ]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

@@ -62,13 +62,23 @@ inline bool is_pow2(unsigned n) { return n != 0 && (n & (n - 1)) == 0; }
 
 // ---- device memory ---------------------------------------------------------
 // Grow-only device buffer.  Growing frees the old allocation; hipFree waits
-// for the device, so no kernel still reads it.
+// for the device, so no kernel still reads it.  A fixed buffer wraps memory
+// the library does not own (a region of the allocation-free scratch): it
+// never allocates or frees, and a request beyond its size is an error.
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    bool fixed = false;
     int ensure(size_t n);
     void release();
     float2 *f2() const { return static_cast<float2 *>(p); }
+    static DevBuf wrap(void *mem, size_t n) {
+        DevBuf b;
+        b.p = mem;
+        b.bytes = n;
+        b.fixed = true;
+        return b;
+    }
 };
 
 // Where a buffer lives: device memory (used in place), pinned host memory

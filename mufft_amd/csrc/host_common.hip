@@ -20,6 +20,10 @@ void set_err(const char *fmt, ...) {
 
 int DevBuf::ensure(size_t n) {
     if (n <= bytes) return 0;
+    if (fixed) {
+        set_err("scratch of %zu bytes is too small (%zu needed)", bytes, n);
+        return -1;
+    }
     release();
     if (hipMalloc(&p, n) != hipSuccess) {
         (void)hipGetLastError();
@@ -32,7 +36,7 @@ int DevBuf::ensure(size_t n) {
 }
 
 void DevBuf::release() {
-    if (p) (void)hipFree(p);
+    if (p && !fixed) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
 }
