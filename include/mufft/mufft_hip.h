@@ -158,8 +158,12 @@ int mufft_hip_tune_load(const char *path);
  *           MUFFT_CONV_METHOD_FLAG_ZERO_PAD_UPPER_HALF_FIRST bit, else N;
  *   second: N/2 reals with ..._SECOND, else N reals;
  *   output: N reals (mono) or N complex (stereo).
- * Mono N = 8..65536, stereo N = 8..32768, powers of two.  The batch owns the
- * spectra scratch; the job pointers are fixed at creation. */
+ * Mono N = 8..65536, stereo N = 8..32768, powers of two.  A batch is a single
+ * kernel launch: each job group is assigned to one XCD, whose workgroups run
+ * its forward transforms and then its inverse transforms as soon as the
+ * group's spectra are ready.  The batch owns the spectra scratch; the job
+ * pointers are fixed at creation, and executions of one batch must not
+ * overlap in time. */
 typedef struct mufft_hip_conv_job {
     unsigned N;
     unsigned method;
@@ -174,9 +178,9 @@ mufft_hip_conv_batch *mufft_hip_create_conv_batch(const mufft_hip_conv_job *jobs
                                                   unsigned flags);
 /* Replays the captured graph on stream (NULL: the batch's stream, and waits). */
 int mufft_hip_execute_conv_batch(mufft_hip_conv_batch *batch, void *stream);
-/* Same kernels without the graph. */
+/* Same kernel without the graph. */
 int mufft_hip_execute_conv_batch_direct(mufft_hip_conv_batch *batch, void *stream);
-/* 0 = the last execution ran every task, 1 = it did not (synchronises the device). */
+/* 0 = every execution so far ran every task, 1 = one did not (synchronises the device). */
 int mufft_hip_conv_batch_status(mufft_hip_conv_batch *batch);
 int mufft_hip_conv_batch_info(const mufft_hip_conv_batch *batch, int *ntasks, int *grid);
 void mufft_hip_free_conv_batch(mufft_hip_conv_batch *batch);

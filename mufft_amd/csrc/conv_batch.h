@@ -22,6 +22,9 @@ struct CBJob {
     float2 *out;           // N reals (mono) or N complex (stereo)
 };
 
+// Dies (XCDs) of the GPU; each has its own L2.
+constexpr int CB_XCDS = 8;
+
 // A group of up to CB_WG / T jobs sharing one transform shape.
 struct CBTask {
     int type;              // CBTaskType
@@ -37,20 +40,41 @@ struct CBTask {
     // (k1 = 0, 1) of the split transform (split_impl.h); stw is its twiddle table.
     int k1;
     const float2 *stw;
+    int group;             // readiness counter of the task's job group
+    int nprod;             // inverse tasks: forward tasks the group needs first
 };
 
-// Tasks [0, nfwd) are the forward transforms, [nfwd, ntasks) the inverse
-// transforms; each phase is one persistent launch with its own queue head.
+// One persistent launch runs the whole batch.  Each job group (the jobs of
+// one task shape) is assigned to one XCD, and that XCD's workgroups (read
+// from the hardware XCC_ID register) pull its tasks from the XCD's own
+// queue: first the forward tasks, then the inverse tasks.  A forward task
+// waits until its stores are complete (they are then in the XCD's L2) and
+// counts itself into the group's readiness counter; an inverse task waits for
+// the count, invalidates its CU's L1 and reads the spectra from the shared
+// L2.  A producer and its consumer never sit on different XCDs, so no
+// device-wide cache write-back is needed (the agent-scope release that would
+// order them across XCDs, buffer_wbl2, costs about as much as a task on
+// gfx950).  An inverse task only waits for forward tasks dequeued before it
+// by running workgroups, so the waits always end; they are bounded all the
+// same.  The last workgroup to finish clears the queue and readiness
+// counters for the next launch.
 struct CBArgs {
     const CBTask *tasks;
-    int nfwd, ntasks;
+    int qbegin[CB_XCDS + 1];      // tasks of XCD x: [qbegin[x], qbegin[x + 1])
     const CBJob *jobs;
     int njobs;
-    int *head;             // queue heads of the two phases
-    int *done;             // tasks completed (both phases)
+    int ngroups;
+    unsigned *head;               // ticket counter per XCD
+    unsigned *ready;              // per job group: forward tasks completed
+    unsigned long long *finish;   // workgroups finished (all launches)
+    unsigned long long *done;     // tasks completed (all launches)
+    unsigned long long *timeouts; // readiness waits that gave up (the batch then failed)
+    unsigned spin_limit;          // polls of a readiness counter before giving up
 };
 
 hipError_t launch_conv_batch(const CBArgs &a, int grid, hipStream_t s);
+// Workgroups of the persistent kernel that stay resident at once.
+int cb_resident_workgroups();
 int cb_jobs_per_task(int logn);
 int cb_points_per_thread(int logn);   // of the (sub-)transform a task runs
 

@@ -3,17 +3,13 @@
 // The reference runs one convolution as three calls (fft.c:799-809): forward
 // transform of each input block, then multiply + inverse transform.  Here a
 // whole batch of such jobs, of any mix of sizes and of both methods (mono:
-// r2c / r2c / c2r, stereo: c2c / full r2c / c2c, fft.c:583-600), runs in two
-// persistent launches: one over all forward tasks (both inputs of every job),
-// one over all inverse tasks (with the spectrum multiply fused into their
-// prologue).  Tasks of equal shape are grouped up to one workgroup's
-// capacity and each phase is ordered longest-first (LPT); workgroups pull
-// tasks from a device queue (an atomic counter).  The launch boundary orders
-// the phases.  A device-wide wait inside one launch would need agent-scope
-// release/acquire fences, which on gfx950 write back or invalidate a whole
-// XCD's L2 (buffer_wbl2 / buffer_inv) at every task and slowed the batch
-// five-fold; the queue atomics here are relaxed and no workgroup ever waits
-// for another.
+// r2c / r2c / c2r, stereo: c2c / full r2c / c2c, fft.c:583-600), runs in one
+// persistent launch: forward tasks (both inputs of every job) and inverse
+// tasks (the spectrum product fused into their prologue), each inverse task
+// starting as soon as its own job group's forward tasks are done.  Tasks of
+// equal shape are grouped up to one workgroup's capacity, job groups are
+// spread over the XCDs longest-first, and each XCD's workgroups pull its
+// tasks from its own queue (a ticket counter); see CBArgs for the ordering.
 #include "conv_batch.h"
 #include "rows_impl.h"
 
@@ -100,49 +96,101 @@ __device__ __forceinline__ void cb_dispatch_sizes(const CBTask &t, const CBJob *
     });
 }
 
-__global__ void __launch_bounds__(CB_WG) conv_batch_kernel(CBArgs a, int phase) {
+template <int KIND_, int DIR_>
+__device__ __forceinline__ void cb_dispatch_kind(const CBTask &t, const CBJob *jobs, int tid, float2 *smem) {
+    cb_dispatch_sizes<KIND_, DIR_, CB_MIN_LOG2, 15>(t, jobs, tid, smem);
+}
+
+__device__ __forceinline__ void cb_run_task(const CBTask &t, const CBJob *jobs, int tid, float2 *smem) {
+    if (t.kind == KIND_R2C) cb_dispatch_kind<KIND_R2C, -1>(t, jobs, tid, smem);
+    else if (t.kind == KIND_C2R) cb_dispatch_kind<KIND_C2R, 1>(t, jobs, tid, smem);
+    else if (t.dir < 0) cb_dispatch_kind<KIND_C2C, -1>(t, jobs, tid, smem);
+    else cb_dispatch_kind<KIND_C2C, 1>(t, jobs, tid, smem);
+}
+
+// XCD (die) this workgroup runs on: the hardware XCC_ID register, bits 3:0.
+__device__ __forceinline__ int cb_xcd() {
+    return __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & (CB_XCDS - 1);
+}
+
+// Each iteration keeps all of thread 0's work (publishing the previous
+// task, the next ticket, the readiness wait) in one region at its top,
+// followed by barriers, so the loop compiles to one uniform loop (a second
+// thread-0 region at the end of the body let the compiler's control-flow
+// structurization send the other lanes of wave 0 back to the barriers ahead
+// of thread 0).
+__global__ void __launch_bounds__(CB_WG) conv_batch_kernel(CBArgs a) {
     __shared__ float2 smem[CB_LDS_BYTES / 8];
-    __shared__ int s_task;
+    __shared__ int s_local;
     const int tid = threadIdx.x;
-    const int begin = phase == 0 ? 0 : a.nfwd;
-    const int end = phase == 0 ? a.nfwd : a.ntasks;
+    const int x = cb_xcd();
+    const int qb = a.qbegin[x], qn = a.qbegin[x + 1] - qb;
+    int prev_group = -1;          // forward task whose completion is to be published
+    bool finished_one = false;
     for (;;) {
-        if (tid == 0) s_task = begin + atomicAdd(a.head + phase, 1);
+        if (tid == 0) {
+            if (prev_group >= 0)
+                __hip_atomic_fetch_add(a.ready + prev_group, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (finished_one) atomicAdd(a.done, 1ull);
+            const int local = (int)atomicAdd(a.head + x, 1u);
+            s_local = local;
+            if (local < qn) {
+                const CBTask &t = a.tasks[qb + local];
+                if (t.type == CB_INV) {
+                    const unsigned need = (unsigned)t.nprod;
+                    unsigned polls = 0;
+                    while (__hip_atomic_load(a.ready + t.group, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+                        if (++polls > a.spin_limit) {
+                            atomicAdd(a.timeouts, 1ull);
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(2);
+                    }
+                    // Drop stale lines of this CU's L1: the spectra are read from the L2.
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                }
+            }
+        }
         __syncthreads();
-        const int ti = s_task;
+        const int local = __builtin_amdgcn_readfirstlane(s_local);
         __syncthreads();
-        if (ti >= end) break;
-        const CBTask t = a.tasks[ti];
-        if (t.kind == KIND_R2C) cb_dispatch_sizes<KIND_R2C, -1, CB_MIN_LOG2, 15>(t, a.jobs, tid, smem);
-        else if (t.kind == KIND_C2R) cb_dispatch_sizes<KIND_C2R, 1, CB_MIN_LOG2, 15>(t, a.jobs, tid, smem);
-        else if (t.dir < 0) cb_dispatch_sizes<KIND_C2C, -1, CB_MIN_LOG2, 15>(t, a.jobs, tid, smem);
-        else cb_dispatch_sizes<KIND_C2C, 1, CB_MIN_LOG2, 15>(t, a.jobs, tid, smem);
-        if (tid == 0) atomicAdd(a.done, 1);
+        if (local >= qn) break;
+        const CBTask t = a.tasks[qb + local];
+        cb_run_task(t, a.jobs, tid, smem);
+        // Forward task: every thread's spectrum stores are complete (in this
+        // XCD's L2) before thread 0 publishes the task at the next iteration.
+        __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
+        __syncthreads();
+        prev_group = t.type == CB_INV ? -1 : t.group;
+        finished_one = true;
+    }
+    // The last workgroup out clears the queue and readiness counters for the
+    // next launch (every other workgroup has made its last access to them).
+    if (tid == 0) {
+        const unsigned long long f = atomicAdd(a.finish, 1ull);
+        if ((f + 1) % gridDim.x == 0) {
+            for (int i = 0; i < CB_XCDS; ++i) __hip_atomic_store(a.head + i, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            for (int g = 0; g < a.ngroups; ++g)
+                __hip_atomic_store(a.ready + g, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
     }
 }
 
-// Clears the queue heads and the completion count (a kernel, so the reset can
-// live inside a captured graph).
-__global__ void __launch_bounds__(64) conv_batch_reset_kernel(CBArgs a) {
-    if (threadIdx.x < 2) a.head[threadIdx.x] = 0;
-    if (threadIdx.x == 0) *a.done = 0;
+int cb_resident_workgroups() {
+    int dev = 0, ncu = 0, occ = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, conv_batch_kernel, CB_WG, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return ncu * occ;
 }
 
 hipError_t launch_conv_batch(const CBArgs &a, int grid, hipStream_t s) {
-    hipLaunchKernelGGL(conv_batch_reset_kernel, dim3(1), dim3(64), 0, s, a);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    const int g0 = a.nfwd < grid ? a.nfwd : grid;
-    const int g1 = a.ntasks - a.nfwd < grid ? a.ntasks - a.nfwd : grid;
-    if (g0 > 0) {
-        hipLaunchKernelGGL(conv_batch_kernel, dim3(g0), dim3(CB_WG), 0, s, a, 0);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-    }
-    if (g1 > 0) {
-        hipLaunchKernelGGL(conv_batch_kernel, dim3(g1), dim3(CB_WG), 0, s, a, 1);
-        e = hipGetLastError();
-    }
-    return e;
+    if (grid <= 0) return hipSuccess;
+    hipLaunchKernelGGL(conv_batch_kernel, dim3(grid), dim3(CB_WG), 0, s, a);
+    return hipGetLastError();
 }
 
 }  // namespace mufft

@@ -62,6 +62,9 @@ def lib():
             ("mufft_free", None, [vp]),
             # oracle/ref_bench.c: the CPU-baseline workload on native threads
             ("mufft_ref_bench_run", ctypes.c_double, [i, i, u, vp, i, u, vp, i]),
+            ("mufft_ref_time_1d", ctypes.c_double, [i, u, i, vp, i]),
+            ("mufft_ref_time_2d", ctypes.c_double, [i, u, u, i, vp, i, i, i]),
+            ("mufft_ref_time_conv", ctypes.c_double, [i, i, i, vp, vp, vp, vp]),
         ]:
             fn = getattr(L, name)
             fn.restype = res

@@ -1,8 +1,10 @@
-"""The uneven convolution batch (two persistent launches, forward then inverse
-tasks, each pulling from a device work queue; hipGraph replay) against FP64
-convolution, direct O(N^2) convolution at small N, and the per-plan drop-in
-path on the same jobs.  Mono N = 65536 and stereo N = 32768 jobs run their
-32768-point transforms as two-workgroup split tasks."""
+"""The uneven convolution batch (one persistent launch: per-XCD work queues,
+each job group's inverse tasks waiting for its forward tasks; hipGraph
+replay) against FP64 convolution, direct O(N^2) convolution at small N, and
+the per-plan drop-in path on the same jobs.  Mono N = 65536 and stereo
+N = 32768 jobs run their 32768-point transforms as two-workgroup split
+tasks.  Replays must give identical bits: the queue and readiness counters
+are cleared inside the kernel by the last workgroup out."""
 import numpy as np
 import pytest
 
@@ -61,7 +63,7 @@ def test_uneven_batch_graph_and_direct(gpu):
     for j in jobs:
         j[4].zero_()
     torch.cuda.synchronize()
-    b.execute()                     # second replay: queue heads were reset inside the graph
+    b.execute()                     # second replay: the ticket queue needs no reset
     assert b.status() == 0
     for j, f in zip(jobs, first):
         np.testing.assert_array_equal(from_dev(j[4]).view(np.uint32), f.view(np.uint32))
