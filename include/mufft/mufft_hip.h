@@ -145,7 +145,31 @@ int mufft_hip_set_variant(unsigned N, int kind, int variant);
  * Each timed launch starts with cold caches (a 256 MiB stream evicts the
  * Infinity Cache first), as a batch streamed from HBM would meet them. */
 int mufft_hip_autotune_1d(unsigned N, int kind, unsigned howmany, double *best_us);
-/* Tune cache as text, one "log2n kind variant us" line per tuned size. */
+/* Transforms beyond the fused sizes (2^17..2^26 complex points; N as for
+ * the plan creators, kind as above) run as two or three column launches over
+ * a split N = N1 x N2 or N1 x A x B, coded log2 N1 << 8 | log2 A << 4 |
+ * log2 B (A = B = 1 for two launches).  The autotuner times every runnable
+ * split at howmany transforms, checks each against an FP64 FFT first
+ * (relative RMS <= 3e-6) and selects the fastest; plans created afterwards
+ * use it (process-wide, as the variants).  No counterpart in the reference,
+ * whose large transforms run its fixed radix passes (README.md:49-51). */
+int mufft_hip_autotune_large_1d(unsigned N, int kind, unsigned howmany, double *best_us);
+int mufft_hip_get_large_split(unsigned N, int kind);
+/* code 0 restores the static split. */
+int mufft_hip_set_large_split(unsigned N, int kind, int code);
+/* 2D column passes: the fused column kernel of length Ny owns tiles of
+ * adjacent columns; two tile widths are compiled per length from 128 points
+ * on.  The autotuner times both on an Nx x Ny image batch of the given kind
+ * (real Nx for r2c / c2r, whose columns are Nx/2 + 1), requires identical
+ * bits and selects the faster for plans created afterwards.  Returns the
+ * width.  The reference picks its vertical passes by a fixed rule
+ * (fft.c:382-435). */
+int mufft_hip_autotune_2d(unsigned Nx, unsigned Ny, int kind, unsigned howmany, double *best_us);
+int mufft_hip_get_column_tile(unsigned L);
+/* tile 0 restores the static width. */
+int mufft_hip_set_column_tile(unsigned L, int tile);
+/* Tune cache as text, one "log2n kind choice us" line per tuned entry (kind
+ * 0-3 fused variants, 4 large-transform splits, 5 column tile widths). */
 int mufft_hip_tune_save(const char *path);
 int mufft_hip_tune_load(const char *path);
 

@@ -60,6 +60,12 @@ SIGNATURES = [
     ("mufft_hip_get_variant", _i, [_u, _i]),
     ("mufft_hip_set_variant", _i, [_u, _i, _i]),
     ("mufft_hip_autotune_1d", _i, [_u, _i, _u, ctypes.POINTER(ctypes.c_double)]),
+    ("mufft_hip_autotune_large_1d", _i, [_u, _i, _u, ctypes.POINTER(ctypes.c_double)]),
+    ("mufft_hip_get_large_split", _i, [_u, _i]),
+    ("mufft_hip_set_large_split", _i, [_u, _i, _i]),
+    ("mufft_hip_autotune_2d", _i, [_u, _u, _i, _u, ctypes.POINTER(ctypes.c_double)]),
+    ("mufft_hip_get_column_tile", _i, [_u]),
+    ("mufft_hip_set_column_tile", _i, [_u, _i]),
     ("mufft_hip_tune_save", _i, [ctypes.c_char_p]),
     ("mufft_hip_tune_load", _i, [ctypes.c_char_p]),
     ("mufft_hip_execute_adjoint_1d_batched", _i, [_vp, _vp, _vp, _u, _sz, _sz, _vp]),

@@ -441,5 +441,44 @@ def get_variant(N: int, kind: str) -> int:
     return int(_lib.load().mufft_hip_get_variant(N, KIND_CODES[kind]))
 
 
+def autotune_large(N: int, kind: str, howmany: int = 1) -> tuple[int, float]:
+    """Times every runnable split of a transform beyond the fused sizes
+    (2^17..2^26 complex points), keeps the fastest that passes the FP64 check
+    for plans created afterwards, and returns (split code, microseconds)."""
+    us = ctypes.c_double(0)
+    c = _lib.load().mufft_hip_autotune_large_1d(N, KIND_CODES[kind], howmany, ctypes.byref(us))
+    if c < 0:
+        raise MufftError(f"autotune_large({N}, {kind}): {_lib.last_error()}")
+    return c, us.value
+
+
+def set_large_split(N: int, kind: str, code: int) -> None:
+    """Selects a split code (0 = static) for plans created afterwards."""
+    _check(_lib.load().mufft_hip_set_large_split(N, KIND_CODES[kind], code), "mufft_hip_set_large_split")
+
+
+def get_large_split(N: int, kind: str) -> int:
+    return int(_lib.load().mufft_hip_get_large_split(N, KIND_CODES[kind]))
+
+
+def autotune_2d(Nx: int, Ny: int, kind: str, howmany: int = 1) -> tuple[int, float]:
+    """Times the column tile widths of the Ny-point column kernel on an
+    Nx x Ny image batch, keeps the faster for plans created afterwards and
+    returns (tile width, microseconds)."""
+    us = ctypes.c_double(0)
+    t = _lib.load().mufft_hip_autotune_2d(Nx, Ny, KIND_CODES[kind], howmany, ctypes.byref(us))
+    if t < 0:
+        raise MufftError(f"autotune_2d({Nx}, {Ny}, {kind}): {_lib.last_error()}")
+    return t, us.value
+
+
+def set_column_tile(L: int, tile: int) -> None:
+    _check(_lib.load().mufft_hip_set_column_tile(L, tile), "mufft_hip_set_column_tile")
+
+
+def get_column_tile(L: int) -> int:
+    return int(_lib.load().mufft_hip_get_column_tile(L))
+
+
 def device_count() -> int:
     return int(_lib.load().mufft_hip_device_count())

@@ -1,5 +1,6 @@
 // Autotuner of the fused row kernels and the tune cache.
 #include <stdio.h>
+#include <string.h>
 
 #include <algorithm>
 #include <cmath>
@@ -75,6 +76,9 @@ std::vector<cd> reference64(int tk, int nc, const float2 *in) {
     return y;
 }
 
+// tk: TuneKind of a row variant, or one of these for the other tuned choices.
+constexpr int TK_SPLIT = 4;     // split of a transform beyond the fused sizes (fs_code)
+constexpr int TK_COLTILE = 5;   // tile width of the column kernel
 struct TuneRecord {
     int logn, tk, variant;
     double us;
@@ -269,6 +273,283 @@ int mufft_hip_autotune_1d(unsigned N, int kind, unsigned howmany, double *best_u
     return guarded<int>(-1, [&] { return mufft_hip_autotune_1d_impl(N, kind, howmany, best_us); });
 }
 
+// ===========================================================================
+// Large transforms: the split N = N1 x N2 (two column launches) or
+// N1 x A x B (three) is chosen by timing every runnable split at the
+// caller's batch, each checked against an FP64 FFT (relative RMS <= 3e-6)
+// first.  The static choice (fs_code_default) stays in force until then.
+// ===========================================================================
+static void record_tune(int l, int tk, int choice, double us) {
+    std::lock_guard<std::mutex> g(g_tune_lock);
+    g_tune_records.erase(std::remove_if(g_tune_records.begin(), g_tune_records.end(),
+                                        [&](const TuneRecord &t) { return t.logn == l && t.tk == tk; }),
+                         g_tune_records.end());
+    g_tune_records.push_back({l, tk, choice, us});
+}
+
+// Complex log2 length of an API size beyond the fused row sizes.
+static int large_logn(unsigned N, int kind) {
+    const int tk = tk_from_api(kind);
+    if (tk < 0 || !is_pow2(N)) return -1;
+    const int l = ilog2(N) - ((tk == TK_R2C || tk == TK_C2R) ? 1 : 0);
+    return (l > ROW_MAX_LOG2 && l <= 2 * COL_MAX_LOG2) ? l : -1;
+}
+
+static int mufft_hip_autotune_large_1d_impl(unsigned N, int kind, unsigned howmany, double *best_us) {
+    const int l = large_logn(N, kind);
+    if (l < 0 || howmany == 0) {
+        set_err("no split to tune for this size (2^17..2^26 complex points)");
+        return -1;
+    }
+    const int tk = tk_from_api(kind);
+    const int dir = (tk == TK_C2C_FWD || tk == TK_R2C) ? -1 : 1;
+    const long long nc = 1LL << l;
+    std::vector<int> codes;
+    for (int n1 = 6; n1 <= 13; ++n1) {
+        if (fs_code_valid(l, n1 << 8)) codes.push_back(n1 << 8);
+        for (int a = 6; a <= 10; ++a) {
+            const int b = l - n1 - a;
+            const int c = (n1 << 8) | (a << 4) | (b > 0 ? b : 0);
+            if (b >= 6 && b <= 10 && fs_code_valid(l, c)) codes.push_back(c);
+        }
+    }
+    // One transform for the FP64 check, the batch for the timing.
+    std::vector<float2> h_in((size_t)nc);
+    uint64_t st = 0x2545F4914F6CDD1Dull ^ (uint64_t)l;
+    for (auto &z : h_in) {
+        st = st * 6364136223846793005ull + 1442695040888963407ull;
+        z.x = (float)((double)(st >> 40) / 16777216.0 - 0.5);
+        st = st * 6364136223846793005ull + 1442695040888963407ull;
+        z.y = (float)((double)(st >> 40) / 16777216.0 - 0.5);
+    }
+    std::vector<cd> ref(h_in.size());
+    for (size_t i = 0; i < h_in.size(); ++i) ref[i] = cd(h_in[i].x, h_in[i].y);
+    fft64(ref, dir);
+    double ref_norm = 0;
+    for (const cd &z : ref) ref_norm += std::norm(z);
+
+    hipStream_t s = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    DevBuf din, dout, s0, s1;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const size_t bytes = (size_t)nc * howmany * sizeof(float2);
+    int rc = 0;
+    if (din.ensure(bytes) || dout.ensure(bytes) || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess) {
+        rc = -1;
+    }
+    for (unsigned b = 0; !rc && b < howmany; ++b)
+        if (hipMemcpy(din.f2() + (size_t)b * nc, h_in.data(), nc * sizeof(float2), hipMemcpyHostToDevice) !=
+            hipSuccess)
+            rc = -1;
+    int best = -1;
+    double best_t = 1e30;
+    std::vector<float2> h_out((size_t)nc);
+    for (size_t i = 0; i < codes.size() && !rc; ++i) {
+        Engine e;
+        if (e.init((int)nc, dir, KIND_C2C, true, false, s, 0, codes[i])) {
+            e.release();
+            continue;
+        }
+        auto run = [&] {
+            return run_rows(e, KIND_C2C, nullptr, din.f2(), nc, nullptr, 0, dout.f2(), nc, (int)howmany, 0u, 1.0f,
+                            s0, s1, s);
+        };
+        bool ok = run() == 0 &&
+                  hipMemcpyAsync(h_out.data(), dout.p, nc * sizeof(float2), hipMemcpyDeviceToHost, s) ==
+                      hipSuccess &&
+                  hipStreamSynchronize(s) == hipSuccess;
+        if (ok) {
+            double num = 0;
+            for (size_t k = 0; k < h_out.size(); ++k) num += std::norm(cd(h_out[k].x, h_out[k].y) - ref[k]);
+            ok = std::sqrt(num / (ref_norm > 0 ? ref_norm : 1.0)) <= 3e-6;
+        }
+        if (ok) {
+            const int reps = 3;
+            double total_ms = 0;
+            for (int r = 0; r < reps && ok; ++r) {
+                (void)hipEventRecord(e0, s);
+                ok = run() == 0;
+                (void)hipEventRecord(e1, s);
+                float ms = 0;
+                ok = ok && hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+                total_ms += ms;
+            }
+            const double us = 1e3 * total_ms / reps;
+            if (ok && us < best_t) {
+                best_t = us;
+                best = codes[i];
+            }
+        }
+        (void)hipStreamSynchronize(s);
+        e.release();
+    }
+    din.release();
+    dout.release();
+    s0.release();
+    s1.release();
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(s);
+    if (rc || best < 0) {
+        if (!rc) set_err("no split passed validation");
+        return -1;
+    }
+    set_fs_code(l, best);
+    record_tune(l, TK_SPLIT, best, best_t);
+    if (best_us) *best_us = best_t;
+    return best;
+}
+
+int mufft_hip_autotune_large_1d(unsigned N, int kind, unsigned howmany, double *best_us) {
+    return guarded<int>(-1, [&] { return mufft_hip_autotune_large_1d_impl(N, kind, howmany, best_us); });
+}
+
+int mufft_hip_get_large_split(unsigned N, int kind) {
+    const int l = large_logn(N, kind);
+    if (l < 0) {
+        set_err("no split for this size");
+        return -1;
+    }
+    return chosen_fs_code(l);
+}
+
+int mufft_hip_set_large_split(unsigned N, int kind, int code) {
+    const int l = large_logn(N, kind);
+    if (l < 0 || (code != 0 && !fs_code_valid(l, code))) {
+        set_err("unknown size or split");
+        return -1;
+    }
+    set_fs_code(l, code);
+    return 0;
+}
+
+// ===========================================================================
+// 2D column passes: the tile width of the column kernel (col_TX or its
+// alternative col_TX_alt) is chosen by timing both on the caller's image
+// shape.  The width changes only which columns a workgroup owns, so both
+// must give identical bits; a width whose output differs is refused.
+// ===========================================================================
+static int mufft_hip_autotune_2d_impl(unsigned Nx, unsigned Ny, int kind, unsigned howmany, double *best_us) {
+    const int tk = tk_from_api(kind);
+    if (tk < 0 || !is_pow2(Nx) || !is_pow2(Ny) || Ny < 2 || howmany == 0 ||
+        ilog2(Ny) > COL_MAX_LOG2 || ((tk == TK_R2C || tk == TK_C2R) && Nx < 4)) {
+        set_err("no column kernel to tune for this shape");
+        return -1;
+    }
+    const int ly = ilog2(Ny);
+    const int alt = col_TX_alt((int)Ny);
+    const int dir = (tk == TK_C2C_FWD || tk == TK_R2C) ? -1 : 1;
+    // Columns and row stride as the 2D plans lay them out (fft.c:664-792):
+    // Nx complex columns per row, of which Nx/2 + 1 are transformed for r2c / c2r.
+    const long long rs = (tk == TK_R2C || tk == TK_C2R) ? Nx / 2 : Nx;
+    const int ncols = (int)((tk == TK_R2C || tk == TK_C2R) ? Nx / 2 + 1 : Nx);
+    const long long img = (tk == TK_R2C || tk == TK_C2R) ? (long long)Nx * Ny : rs * Ny;
+    const long long rstride = (tk == TK_R2C || tk == TK_C2R) ? (long long)Nx : rs;
+    hipStream_t s = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    DevBuf din, dout, alt_buf;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const size_t bytes = (size_t)img * howmany * sizeof(float2);
+    int rc = 0;
+    std::vector<float2> h_in((size_t)img);
+    uint64_t st = 0x9E3779B97F4A7C15ull ^ (uint64_t)Ny;
+    for (auto &z : h_in) {
+        st = st * 6364136223846793005ull + 1442695040888963407ull;
+        z.x = (float)((double)(st >> 40) / 16777216.0 - 0.5);
+        z.y = (float)((double)(st & 0xffffff) / 16777216.0 - 0.5);
+    }
+    if (din.ensure(bytes) || dout.ensure(bytes) || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess)
+        rc = -1;
+    for (unsigned b = 0; !rc && b < howmany; ++b)
+        if (hipMemcpy(din.f2() + (size_t)b * img, h_in.data(), img * sizeof(float2), hipMemcpyHostToDevice) !=
+            hipSuccess)
+            rc = -1;
+    const int cands[2] = {col_TX((int)Ny), alt};
+    std::vector<float2> first, h_out((size_t)img);
+    int best = -1;
+    double best_t = 1e30;
+    for (int ci = 0; ci < 2 && !rc; ++ci) {
+        const int tx = cands[ci];
+        if (tx <= 0) continue;
+        Engine e;
+        if (e.init((int)Ny, dir, KIND_C2C, false, true, s, 0, 0, tx)) {
+            e.release();
+            continue;
+        }
+        auto run = [&] {
+            return run_cols(e, din.f2(), rstride, img, dout.f2(), rstride, img, ncols, (int)howmany, alt_buf, s);
+        };
+        bool ok = run() == 0 &&
+                  hipMemcpyAsync(h_out.data(), dout.p, img * sizeof(float2), hipMemcpyDeviceToHost, s) ==
+                      hipSuccess &&
+                  hipStreamSynchronize(s) == hipSuccess;
+        if (ok && first.empty()) first = h_out;
+        else if (ok) {
+            for (long long r = 0; r < Ny && ok; ++r)
+                ok = memcmp(&h_out[(size_t)(r * rstride)], &first[(size_t)(r * rstride)],
+                                 (size_t)ncols * sizeof(float2)) == 0;
+        }
+        if (ok) {
+            const int reps = 5;
+            double total_ms = 0;
+            (void)run();
+            for (int r = 0; r < reps && ok; ++r) {
+                (void)hipEventRecord(e0, s);
+                ok = run() == 0;
+                (void)hipEventRecord(e1, s);
+                float ms = 0;
+                ok = ok && hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+                total_ms += ms;
+            }
+            const double us = 1e3 * total_ms / reps;
+            if (ok && us < best_t) {
+                best_t = us;
+                best = tx;
+            }
+        }
+        (void)hipStreamSynchronize(s);
+        e.release();
+    }
+    din.release();
+    dout.release();
+    alt_buf.release();
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(s);
+    if (rc || best < 0) {
+        if (!rc) set_err("no tile width passed");
+        return -1;
+    }
+    set_col_tx(ly, best);
+    record_tune(ly, TK_COLTILE, best, best_t);
+    if (best_us) *best_us = best_t;
+    return best;
+}
+
+int mufft_hip_autotune_2d(unsigned Nx, unsigned Ny, int kind, unsigned howmany, double *best_us) {
+    return guarded<int>(-1, [&] { return mufft_hip_autotune_2d_impl(Nx, Ny, kind, howmany, best_us); });
+}
+
+int mufft_hip_get_column_tile(unsigned L) {
+    if (!is_pow2(L) || L < 2 || ilog2(L) > COL_MAX_LOG2) {
+        set_err("no fused column kernel of this length");
+        return -1;
+    }
+    return chosen_col_tx(ilog2(L));
+}
+
+int mufft_hip_set_column_tile(unsigned L, int tx) {
+    if (!is_pow2(L) || L < 2 || ilog2(L) > COL_MAX_LOG2 ||
+        (tx != 0 && tx != col_TX((int)L) && tx != col_TX_alt((int)L))) {
+        set_err("unknown column length or tile width");
+        return -1;
+    }
+    set_col_tx(ilog2(L), tx);
+    return 0;
+}
+
 // Tune cache: one line per entry, "log2n kind variant microseconds".
 static int mufft_hip_tune_save_impl(const char *path) {
     FILE *f = fopen(path, "w");
@@ -286,6 +567,9 @@ static int mufft_hip_tune_save_impl(const char *path) {
                     if (t.logn == l && t.tk == tk) us = t.us;
                 fprintf(f, "%d %d %d %.3f\n", l, tk, chosen_variant(l, tk), us);
             }
+    // Tuned splits (kind 4, a fs_code) and column tile widths (kind 5).
+    for (const auto &t : g_tune_records)
+        if (t.tk == TK_SPLIT || t.tk == TK_COLTILE) fprintf(f, "%d %d %d %.3f\n", t.logn, t.tk, t.variant, t.us);
     fclose(f);
     return 0;
 }
@@ -307,6 +591,20 @@ static int mufft_hip_tune_load_impl(const char *path) {
         int l, tk, var;
         double us;
         if (line[0] == '#' || sscanf(line, "%d %d %d %lf", &l, &tk, &var, &us) != 4) continue;
+        if (tk == TK_SPLIT) {
+            if (l <= ROW_MAX_LOG2 || l > 2 * COL_MAX_LOG2 || !fs_code_valid(l, var)) continue;
+            set_fs_code(l, var);
+            record_tune(l, tk, var, us);
+            ++n;
+            continue;
+        }
+        if (tk == TK_COLTILE) {
+            if (l < 1 || l > COL_MAX_LOG2 || (var != col_TX(1 << l) && var != col_TX_alt(1 << l))) continue;
+            set_col_tx(l, var);
+            record_tune(l, tk, var, us);
+            ++n;
+            continue;
+        }
         if (l < 1 || l > ROW_MAX_LOG2 || tk < 0 || tk > 3 || !row_variant_registered(l, var)) continue;
         set_chosen_variant(l, tk, var);
         ++n;

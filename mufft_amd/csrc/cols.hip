@@ -13,12 +13,13 @@
 
 namespace mufft {
 
-template <int L_, int DIR, int MODE = COL_PLAIN>
-__global__ void __launch_bounds__(col_WG(L_)) fft_cols_kernel(ColArgs a) {
+// TXP: tile width (columns per workgroup); 0 = the default col_TX(L).
+template <int L_, int DIR, int MODE = COL_PLAIN, int TXP = 0>
+__global__ void __launch_bounds__(col_T(L_) * (TXP ? TXP : col_TX(L_))) fft_cols_kernel(ColArgs a) {
     constexpr int E = col_E(L_);
     constexpr int LOGE = ilog2(E);
     constexpr int T = col_T(L_);
-    constexpr int TX = col_TX(L_);
+    constexpr int TX = TXP ? TXP : col_TX(L_);
     constexpr int ROWS = col_lds_rows(L_);
     constexpr int XM = col_xm(L_);
     __shared__ float2 smem[XM == XM_SPLIT ? (ROWS * TX + 1) / 2 : ROWS * TX];
@@ -147,6 +148,21 @@ hipError_t launch_four_step_cols(int logN1, int direction, const FourStepArgs &a
     return hipLaunchKernel(k, dim3((unsigned)nblocks), dim3(fs_WG(L)), params, 0, s);
 }
 
+// The alternative tile width of a column length (col_TX_alt), plain mode.
+template <int DIR>
+static const void *col_alt_kernel_ptr(int logL) {
+    switch (logL) {
+    case 7: return (const void *)&fft_cols_kernel<128, DIR, COL_PLAIN, col_TX_alt(128)>;
+    case 8: return (const void *)&fft_cols_kernel<256, DIR, COL_PLAIN, col_TX_alt(256)>;
+    case 9: return (const void *)&fft_cols_kernel<512, DIR, COL_PLAIN, col_TX_alt(512)>;
+    case 10: return (const void *)&fft_cols_kernel<1024, DIR, COL_PLAIN, col_TX_alt(1024)>;
+    case 11: return (const void *)&fft_cols_kernel<2048, DIR, COL_PLAIN, col_TX_alt(2048)>;
+    case 12: return (const void *)&fft_cols_kernel<4096, DIR, COL_PLAIN, col_TX_alt(4096)>;
+    case 13: return (const void *)&fft_cols_kernel<8192, DIR, COL_PLAIN, col_TX_alt(8192)>;
+    default: return nullptr;
+    }
+}
+
 template <int DIR>
 static const void *col_kernel_ptr(int logL) {
     switch (logL) {
@@ -181,9 +197,16 @@ static const void *col_mode_kernel_ptr(int logL) {
     }
 }
 
-hipError_t launch_cols(int logL, int direction, const ColArgs &a, hipStream_t s, int mode) {
+hipError_t launch_cols(int logL, int direction, const ColArgs &a, hipStream_t s, int mode, int tx) {
     const void *k = nullptr;
-    if (mode == COL_PLAIN) k = direction < 0 ? col_kernel_ptr<-1>(logL) : col_kernel_ptr<1>(logL);
+    const int L = 1 << logL;
+    if (tx == 0) tx = col_TX(L);
+    if (tx != col_TX(L) && (mode != COL_PLAIN || logL < 1 || logL > COL_MAX_LOG2 || tx != col_TX_alt(L)))
+        return hipErrorInvalidValue;
+    if (a.ntiles <= 0 || (long long)a.ntiles * tx < a.ncols) return hipErrorInvalidValue;
+    if (mode == COL_PLAIN && tx != col_TX(L))
+        k = direction < 0 ? col_alt_kernel_ptr<-1>(logL) : col_alt_kernel_ptr<1>(logL);
+    else if (mode == COL_PLAIN) k = direction < 0 ? col_kernel_ptr<-1>(logL) : col_kernel_ptr<1>(logL);
     else if (mode == COL_TWIDDLE)
         k = direction < 0 ? col_mode_kernel_ptr<-1, COL_TWIDDLE>(logL) : col_mode_kernel_ptr<1, COL_TWIDDLE>(logL);
     else if (mode == COL_TWO_LEVEL)
@@ -191,13 +214,12 @@ hipError_t launch_cols(int logL, int direction, const ColArgs &a, hipStream_t s,
     if (!k) return hipErrorInvalidValue;
     if (mode == COL_TWIDDLE && (!a.coarse || !a.fine)) return hipErrorInvalidValue;
     if (mode == COL_TWO_LEVEL && a.img_div <= 0) return hipErrorInvalidValue;
-    const int L = 1 << logL;
     const long long nblocks = (long long)a.ntiles * a.batch;
     if (nblocks <= 0) return hipSuccess;
     if (nblocks > 0x7fffffffLL) return hipErrorInvalidValue;
     ColArgs args = a;
     void *params[] = {&args};
-    return hipLaunchKernel(k, dim3((unsigned)nblocks), dim3(col_WG(L)), params, 0, s);
+    return hipLaunchKernel(k, dim3((unsigned)nblocks), dim3(col_T(L) * tx), params, 0, s);
 }
 
 }  // namespace mufft

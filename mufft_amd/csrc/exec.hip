@@ -226,10 +226,11 @@ int run_cols(const Engine &e, const float2 *src, long long rs_src, long long bs_
         a.out_rs = rs_dst;
         a.in_bs = bs_src;
         a.out_bs = bs_dst;
+        const int tx = e.col_tx ? e.col_tx : col_TX(e.n);
         a.ncols = ncols;
-        a.ntiles = (ncols + col_TX(e.n) - 1) / col_TX(e.n);
+        a.ntiles = (ncols + tx - 1) / tx;
         a.batch = batch;
-        HIP_TRY(launch_cols(e.logn, e.direction, a, st));
+        HIP_TRY(launch_cols(e.logn, e.direction, a, st, COL_PLAIN, tx));
         return 0;
     }
     if (src == dst) {

@@ -146,6 +146,14 @@ MUFFT_HD constexpr int fs_WG(int L) { return col_T(L) * fs_TX(L); }
 // share a CU (2D 4096^2: -5%); at L = 8192 it allows 4 columns instead of 2
 // (2D 8192^2: -6%, four-step 2^26: -19%).
 MUFFT_HD constexpr int col_TX(int L) { return L == 1024 ? 16 : (L == 2048 ? 4 : (L == 8192 ? 4 : fs_TX(L))); }
+// The one alternative tile width compiled per column length (0: none): twice
+// the default below 1024 points, half or twice it from 1024 on where the LDS
+// and the 1024-thread workgroup allow.  The tile width changes only which
+// columns a workgroup owns, never the arithmetic, so every choice gives the
+// same bits; the 2D autotuner times both on the caller's image shape.
+MUFFT_HD constexpr int col_TX_alt(int L) {
+    return L < 128 ? 0 : (L <= 512 ? 2 * col_TX(L) : (L == 1024 ? 8 : (L == 2048 ? 8 : 2)));
+}
 MUFFT_HD constexpr int col_xm(int L) { return L >= 1024 ? 1 : 0; }    // XMode
 MUFFT_HD constexpr int col_twm(int L) { return L >= 1024 ? 1 : 0; }   // TwMode
 MUFFT_HD constexpr int col_WG(int L) { return col_T(L) * col_TX(L); }
@@ -169,9 +177,29 @@ MUFFT_HD constexpr int fs3_split(int logn) {
          : logn == 26 ? (10 << 8) | (9 << 4) | 7
          : 0;
 }
+// Split codes of a large transform (the planner's static choice below, or
+// one the autotuner measured): log2 N1 << 8 | log2 A << 4 | log2 B, with
+// A = B = 1 (code low bits 0) for two launches.
+MUFFT_HD constexpr int fs_code_n1(int code) { return code >> 8; }
+MUFFT_HD constexpr int fs_code_a(int code) { return (code >> 4) & 15; }
+MUFFT_HD constexpr int fs_code_b(int code) { return code & 15; }
+// Whether a split code is runnable for 2^logn points: the first launch's
+// N1-point columns (64..8192) come in whole tiles of the four-step column
+// kernel; two launches need N2 <= 8192 points per column, three launches
+// A and B of 64..1024 points.
+MUFFT_HD constexpr bool fs_code_valid(int logn, int code) {
+    return fs_code_n1(code) >= 6 && fs_code_n1(code) <= 13 && fs_code_n1(code) < logn &&
+           ((1LL << (logn - fs_code_n1(code))) % fs_TX(1 << fs_code_n1(code))) == 0 &&
+           (((code & 0xff) == 0 && logn - fs_code_n1(code) >= 1 && logn - fs_code_n1(code) <= 13) ||
+            ((code & 0xff) != 0 && fs_code_a(code) >= 6 && fs_code_a(code) <= 10 && fs_code_b(code) >= 6 &&
+             fs_code_b(code) <= 10 && fs_code_n1(code) + fs_code_a(code) + fs_code_b(code) == logn));
+}
 MUFFT_HD constexpr int fs_log_n1(int logn) {
     return logn == 17 ? 9 : logn == 18 ? 8 : logn == 21 ? 11 : logn == 22 ? 12 : logn == 23 ? 11
          : logn == 25 ? 13 : logn / 2;
+}
+MUFFT_HD constexpr int fs_code_default(int logn) {
+    return fs3_split(logn) ? fs3_split(logn) : (fs_log_n1(logn) << 8);
 }
 constexpr int ROW_MAX_LOG2 = 16;   // fused row kernels (one or two workgroups per transform) cover N <= 65536
 constexpr int ROW_WG_MAX_LOG2 = 15;  // one workgroup per transform up to 32768 points

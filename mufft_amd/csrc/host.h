@@ -136,6 +136,13 @@ int chosen_variant(int logn, int tk);
 void set_chosen_variant(int logn, int tk, int variant);
 // Whether a size has a tuned or explicitly set choice.
 bool variant_is_set(int logn, int tk);
+// Process-wide split of transforms of 2^logn points beyond the fused sizes
+// (a fs_code; 0 = the static fs_code_default) and tile width of the column
+// kernel of 2^logL points (0 = col_TX): the autotuner's or an explicit choice.
+int chosen_fs_code(int logn);
+void set_fs_code(int logn, int code);
+int chosen_col_tx(int logL);
+void set_col_tx(int logL, int tx);
 
 // ---- one transform axis ----------------------------------------------------
 // A complex transform of length n along one axis: fused row kernel, fused
@@ -155,6 +162,8 @@ struct Engine {
     // Three launches (fs3_split): N2 = A * B column passes, the middle one with
     // the twiddle w_N2^(b*k_a) from its own two-level tables.
     int log_a = 0, log_b = 0, log_s2 = 0;
+    int fs_code = 0;              // the split in use (fs_code_*)
+    int col_tx = 0;               // column tile width (0 = col_TX(n))
     float2 *tw_pa = nullptr, *tw_pb = nullptr, *tw_coarse2 = nullptr, *tw_fine2 = nullptr;
     std::vector<int> gen_radix;
 
@@ -165,8 +174,11 @@ struct Engine {
     int launches_generic() const { return (int)gen_radix.size(); }
 
     // rows / cols: which roles this engine serves; kind: KIND_* of the row
-    // role; var: the row variant (0 = the process-wide choice).
-    int init(int n_, int dir, int kind, bool rows, bool cols, hipStream_t s, int var = 0);
+    // role; var: the row variant (0 = the process-wide choice); fs: the split
+    // beyond the fused sizes (0 = the process-wide choice); tx: the column
+    // tile width (0 = the process-wide choice).
+    int init(int n_, int dir, int kind, bool rows, bool cols, hipStream_t s, int var = 0, int fs = 0,
+             int tx = 0);
     void release();
 };
 

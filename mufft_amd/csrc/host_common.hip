@@ -189,8 +189,41 @@ bool variant_is_set(int logn, int tk) {
     return g_variant[logn][tk] != 0;
 }
 
+namespace {
+constexpr bool default_splits_valid() {
+    for (int l = ROW_MAX_LOG2 + 1; l <= 2 * COL_MAX_LOG2; ++l)
+        if (!fs_code_valid(l, fs_code_default(l))) return false;
+    return true;
+}
+static_assert(default_splits_valid(), "every static split of the four-step sizes is runnable");
+int g_fs_code[2 * COL_MAX_LOG2 + 1] = {};
+int g_col_tx[COL_MAX_LOG2 + 1] = {};
+}  // namespace
+
+int chosen_fs_code(int logn) {
+    std::lock_guard<std::mutex> g(g_var_lock);
+    const int c = (logn >= 0 && logn <= 2 * COL_MAX_LOG2) ? g_fs_code[logn] : 0;
+    return c ? c : fs_code_default(logn);
+}
+
+void set_fs_code(int logn, int code) {
+    std::lock_guard<std::mutex> g(g_var_lock);
+    if (logn >= 0 && logn <= 2 * COL_MAX_LOG2) g_fs_code[logn] = code;
+}
+
+int chosen_col_tx(int logL) {
+    std::lock_guard<std::mutex> g(g_var_lock);
+    const int t = (logL >= 0 && logL <= COL_MAX_LOG2) ? g_col_tx[logL] : 0;
+    return t ? t : col_TX(1 << logL);
+}
+
+void set_col_tx(int logL, int tx) {
+    std::lock_guard<std::mutex> g(g_var_lock);
+    if (logL >= 0 && logL <= COL_MAX_LOG2) g_col_tx[logL] = tx;
+}
+
 // ---- Engine ----------------------------------------------------------------------
-int Engine::init(int n_, int dir, int kind, bool rows, bool cols, hipStream_t s, int var) {
+int Engine::init(int n_, int dir, int kind, bool rows, bool cols, hipStream_t s, int var, int fs, int tx) {
     n = n_;
     logn = ilog2(n_);
     direction = dir;
@@ -201,11 +234,17 @@ int Engine::init(int n_, int dir, int kind, bool rows, bool cols, hipStream_t s,
         if (rv_split(row_var) > 0 && build_split_table(n, n2, dir, &tw_split, s)) return -1;
     }
     if (cols && col_fused()) {
+        col_tx = tx ? tx : chosen_col_tx(logn);
         if (build_stockham_table(n, dir, fused_sequence(n, col_E(n), false), &tw_col, s)) return -1;
     }
     if (rows && four_step()) {
-        const int f3 = fs3_split(logn);
-        log_n1 = f3 ? (f3 >> 8) : fs_log_n1(logn);
+        fs_code = fs ? fs : chosen_fs_code(logn);
+        if (!fs_code_valid(logn, fs_code)) {
+            set_err("split %#x is not valid for 2^%d points", fs_code, logn);
+            return -1;
+        }
+        const int f3 = (fs_code & 0xff) ? fs_code : 0;
+        log_n1 = fs_code_n1(fs_code);
         log_n2 = logn - log_n1;
         log_s = log_n2;
         const int n1 = 1 << log_n1, n2 = 1 << log_n2;

@@ -124,7 +124,8 @@ hipError_t launch_rows(int logN, int direction, int kind, int var, const RowArgs
 // Registered variants of a size; returns the count (at most max are written).
 int row_variant_list(int logN, int *vars, int max);
 bool row_variant_registered(int logN, int var);
-hipError_t launch_cols(int logL, int direction, const ColArgs &a, hipStream_t s, int mode = COL_PLAIN);
+// tx: tile width, col_TX(L) (0) or col_TX_alt(L) (plain mode only); ntiles must cover ncols.
+hipError_t launch_cols(int logL, int direction, const ColArgs &a, hipStream_t s, int mode = COL_PLAIN, int tx = 0);
 hipError_t launch_pass(int radix, int direction, const PassArgs &a, hipStream_t s);
 hipError_t launch_resolve_r2c(const ResolveArgs &a, hipStream_t s);
 hipError_t launch_resolve_c2r(const ResolveArgs &a, hipStream_t s);

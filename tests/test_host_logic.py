@@ -173,3 +173,66 @@ def test_wrappers_refuse_bad_buffers_before_the_c_call():
         q._validate(np.zeros(16 * 8, np.float32), np.zeros((8, 9), np.complex64), 1)
     # raw addresses are passed through unchecked
     p._validate(0x1000, 0x2000, 4, 0, 0)
+
+
+def _code(n1, a=0, b=0):
+    return (n1 << 8) | (a << 4) | b
+
+
+def test_large_splits_static_choice_and_validation(lib):
+    # The static splits of the four-step sizes (a sweep on MI355X): 2^24..2^26
+    # in three launches, 2^17..2^23 in two.
+    assert lib.mufft_hip_get_large_split(1 << 24, 0) == _code(10, 8, 6)
+    assert lib.mufft_hip_get_large_split(1 << 26, 1) == _code(10, 9, 7)
+    assert lib.mufft_hip_get_large_split(1 << 17, 0) == _code(9)
+    assert lib.mufft_hip_get_large_split(1 << 18, 2) == _code(9)   # r2c of 2^18 reals: 2^17 complex
+    # fused sizes and sizes beyond the four-step range have no split
+    for n, kind in ((1 << 16, 0), (1 << 27, 0), (1 << 17, 2), (3 << 17, 0)):
+        assert lib.mufft_hip_get_large_split(n, kind) == -1
+    # unrunnable codes: N1 beyond the four-step column kernel, N2 above 8192,
+    # A or B beyond 1024, factors that do not multiply to N
+    for n, code in ((1 << 20, _code(14)), (1 << 22, _code(8)), (1 << 24, _code(6, 11, 7)),
+                    (1 << 24, _code(10, 8, 7)), (1 << 18, _code(5))):
+        assert lib.mufft_hip_set_large_split(n, 0, code) == -1, (n, hex(code))
+    try:
+        assert lib.mufft_hip_set_large_split(1 << 20, 0, _code(12)) == 0
+        assert lib.mufft_hip_get_large_split(1 << 20, 1) == _code(12)   # keyed by size, both kinds
+    finally:
+        assert lib.mufft_hip_set_large_split(1 << 20, 0, 0) == 0
+    assert lib.mufft_hip_get_large_split(1 << 20, 0) == _code(10)
+
+
+def test_column_tiles_static_choice_and_validation(lib):
+    assert lib.mufft_hip_get_column_tile(1024) == 16
+    assert lib.mufft_hip_get_column_tile(2048) == 4
+    assert lib.mufft_hip_get_column_tile(8192) == 4
+    assert lib.mufft_hip_get_column_tile(16384) == -1
+    assert lib.mufft_hip_set_column_tile(2048, 5) == -1
+    assert lib.mufft_hip_set_column_tile(64, 128) == -1      # no alternative below 128 points
+    try:
+        assert lib.mufft_hip_set_column_tile(2048, 8) == 0
+        assert lib.mufft_hip_get_column_tile(2048) == 8
+    finally:
+        assert lib.mufft_hip_set_column_tile(2048, 0) == 0
+    assert lib.mufft_hip_get_column_tile(2048) == 4
+
+
+def test_tune_cache_carries_splits_and_tiles(lib, tmp_path):
+    f = tmp_path / "tune.txt"
+    f.write_text("# hand-written cache\n"
+                 f"22 4 {_code(12)} 100.0\n"      # 2^22 in two launches, N1 = 4096
+                 "11 5 8 20.5\n"                   # 2048-point columns, 8-column tiles
+                 f"22 4 {_code(14)} 1.0\n"         # not runnable: ignored
+                 "11 5 7 1.0\n")                   # not a compiled width: ignored
+    try:
+        assert lib.mufft_hip_tune_load(str(f).encode()) == 2
+        assert lib.mufft_hip_get_large_split(1 << 22, 0) == _code(12)
+        assert lib.mufft_hip_get_column_tile(2048) == 8
+        g = tmp_path / "again.txt"
+        assert lib.mufft_hip_tune_save(str(g).encode()) == 0
+        lines = [l.split() for l in g.read_text().splitlines() if not l.startswith("#")]
+        assert ["22", "4", str(_code(12))] in [l[:3] for l in lines]
+        assert ["11", "5", "8"] in [l[:3] for l in lines]
+    finally:
+        lib.mufft_hip_set_large_split(1 << 22, 0, 0)
+        lib.mufft_hip_set_column_tile(2048, 0)
