@@ -6,10 +6,13 @@ importing the package raises, so nothing can silently run on the CPU.
 from __future__ import annotations
 
 import ctypes
+import os
 import sys
 from pathlib import Path
 
-LIB_PATH = Path(__file__).resolve().parent / "libmufft_hip.so"
+# MUFFT_AMD_LIBRARY selects another build of the same library (the
+# host-sanitized one of the CPU tests); the default is the in-tree build.
+LIB_PATH = Path(os.environ.get("MUFFT_AMD_LIBRARY") or Path(__file__).resolve().parent / "libmufft_hip.so")
 
 _vp, _u, _i, _f, _sz = ctypes.c_void_p, ctypes.c_uint, ctypes.c_int, ctypes.c_float, ctypes.c_size_t
 

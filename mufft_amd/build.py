@@ -58,5 +58,56 @@ def build(jobs: int | None = None) -> Path:
     return LIB
 
 
+# Host-only units (planner, executor, C ABI): rebuilt with AddressSanitizer
+# and UndefinedBehaviorSanitizer on the host side only (GPU code is compiled
+# unchanged; GPU sanitizers are not used) and linked with the kernel units of
+# the regular build into a separate library that the CPU tests load.
+HOST_UNITS = [s for s in SOURCES if s not in (
+    "rows_c2c_fwd.hip", "rows_c2c_inv.hip", "rows_r2c.hip", "rows_c2r.hip", "conv_batch.hip", "cols.hip",
+    "misc.hip")]
+SAN_LIB = BUILD / "libmufft_hip_san.so"
+SAN_FLAGS = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+             "-Xarch_host", "-fno-sanitize-recover=undefined", "-Xarch_host", "-fno-omit-frame-pointer",
+             "-Xarch_host", "-g"]
+
+
+def asan_runtime() -> Path:
+    """The shared AddressSanitizer runtime the sanitized library needs preloaded."""
+    r = subprocess.run([HIPCC, "-print-file-name=libclang_rt.asan-x86_64.so"], capture_output=True, text=True)
+    return Path(r.stdout.strip())
+
+
+def build_sanitized(jobs: int | None = None) -> Path:
+    BUILD.mkdir(exist_ok=True)
+    kernel_objs = [BUILD / (Path(s).stem + ".o") for s in SOURCES if s not in HOST_UNITS]
+    if not all(o.exists() for o in kernel_objs):
+        build(jobs)
+
+    def compile_san(src: str) -> Path:
+        obj = BUILD / (Path(src).stem + ".san.o")
+        cmd = [HIPCC, *FLAGS, *SAN_FLAGS, "-c", str(CSRC / src), "-o", str(obj)]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError(f"hipcc (sanitized host) failed on {src}")
+        return obj
+
+    jobs = jobs or min(len(HOST_UNITS), max(1, (os.cpu_count() or 2)))
+    with ThreadPoolExecutor(jobs) as ex:
+        objs = list(ex.map(compile_san, HOST_UNITS))
+    tmp = SAN_LIB.with_suffix(".so.tmp")
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-shared-libasan",
+           "-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+           "-o", str(tmp), *map(str, objs), *map(str, kernel_objs)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("sanitized link failed")
+    os.replace(tmp, SAN_LIB)
+    return SAN_LIB
+
+
 if __name__ == "__main__":
     print(build())
+    if "--sanitized" in sys.argv:
+        print(build_sanitized())

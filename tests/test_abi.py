@@ -5,6 +5,7 @@ The shared library loads, exports every function the public headers declare
 refuse invalid sizes with NULL (fft.c:478-481, 628-631, 666-669) before any
 device work.
 """
+import os
 import re
 import subprocess
 from pathlib import Path
@@ -103,5 +104,7 @@ def test_package_refuses_to_import_without_library(tmp_path):
     import sys
     pkg = tmp_path / "mufft_amd"
     shutil.copytree(ROOT / "mufft_amd", pkg, ignore=shutil.ignore_patterns("*.so", "_build", "csrc"))
-    r = subprocess.run([sys.executable, "-c", "import mufft_amd"], cwd=tmp_path, capture_output=True, text=True)
+    env = {k: v for k, v in os.environ.items() if k != "MUFFT_AMD_LIBRARY"}
+    r = subprocess.run([sys.executable, "-c", "import mufft_amd"], cwd=tmp_path, capture_output=True, text=True,
+                       env=env)
     assert r.returncode != 0 and "libmufft_hip.so" in r.stderr
