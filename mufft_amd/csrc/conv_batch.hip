@@ -19,7 +19,7 @@ namespace mufft {
 // points per thread with split twiddles; at 2^CB_SPLIT_LOG2 points the same
 // variant runs each 16384-point half of the split transform.  It fits CB_LDS_BYTES.
 MUFFT_HD constexpr int cb_variant(int N) {
-    return rv_make(16, 0, 1);
+    return rv_with_chunked_loads(rv_make(16, 0, 1));   // the chunks apply to the c2r steps only
 }
 MUFFT_HD constexpr int cb_sub_n(int logn) { return logn >= CB_SPLIT_LOG2 ? (1 << (logn - 1)) : (1 << logn); }
 

@@ -46,7 +46,17 @@ MUFFT_HD constexpr int ilog2(long long n) { return n <= 1 ? 0 : 1 + ilog2(n >> 1
 //   bits 20-23 workgroups per CU to budget registers for (0: no request);
 //   bits 24-27 log2 of the radix of every pass but the remainder pass (0:
 //              radix E); a radix below E gives each thread E/radix
-//              butterflies per pass.
+//              butterflies per pass;
+//   bit  28    c2r: the prologue reads its bins in chunks of 4 points per
+//              thread, each chunk issued after the previous one's data has
+//              arrived, in the order 0-3, E-4..E-1, 4-7, E-8..E-5, ...  Every
+//              bin is read twice (as X[k] by one thread, as X[N - k] by its
+//              mirror thread).  Issued all at once, both reads miss the L2
+//              together and each costs an HBM read; chunk pairs make the
+//              second read follow the first by one load round, so it hits
+//              the line the first brought in (a whole transform's input, 256
+//              KB per CU at 32768 points, does not stay in a 4 MB L2 shared
+//              by 32 CUs for much longer).
 // The planner's static choice is rv_default(); the autotuner may pick any
 // registered variant.
 // ---------------------------------------------------------------------------
@@ -54,6 +64,8 @@ MUFFT_HD constexpr int rv_make(int E, int xm, int twm) { return E | (xm << 8) | 
 MUFFT_HD constexpr int rv_make_split(int log2n1, int var) { return var | (log2n1 << 16); }
 MUFFT_HD constexpr int rv_with_occ(int var, int wgs_per_cu) { return var | (wgs_per_cu << 20); }
 MUFFT_HD constexpr int rv_with_radix(int var, int log2r) { return var | (log2r << 24); }
+MUFFT_HD constexpr int rv_with_chunked_loads(int var) { return var | (1 << 28); }
+MUFFT_HD constexpr bool rv_chunked_loads(int var) { return ((var >> 28) & 1) != 0; }
 MUFFT_HD constexpr int rv_split(int var) { return (var >> 16) & 0xf; }   // log2 N1
 MUFFT_HD constexpr int rv_occ(int var) { return (var >> 20) & 0xf; }
 MUFFT_HD constexpr int rv_sub(int var) { return var & ~(0xf << 16); }    // variant of the sub-transform
@@ -94,13 +106,20 @@ MUFFT_HD constexpr int rv_lds_per(int N, int var) {
 // (c2c and c2r 32 points, r2c 64 points per thread) beats the two-workgroup
 // split by 20-34% on cold data; the split only wins while a batch stays in
 // the Infinity Cache.  65536 runs on two workgroups per transform.
-MUFFT_HD constexpr int rv_default(int N, int kind = 0) {
+MUFFT_HD constexpr int rv_default_base(int N, int kind) {
     return N < 512 ? (kind == 3 && N == 256 ? rv_make(8, 0, 0) : rv_make(16, 0, 0))
          : N == 512 ? (kind == 3 ? rv_make(8, 0, 0) : rv_make(16, 0, 1))
          : N < 16384 ? ((N == 1024 || N == 8192) && kind < 2 ? rv_make(32, 0, 1) : rv_make(16, 0, 1))
          : N == 16384 ? (kind == 3 ? rv_make(32, 0, 1) : rv_with_occ(rv_make(32, 1, 1), 2))
          : N == 32768 ? (kind == 2 ? rv_make(64, 1, 1) : rv_make(32, 1, 1))
          : rv_make_split(1, rv_make(32, 1, 1));
+}
+// c2r from 8192 complex points on reads its bins in mirrored chunks
+// (rv_chunked_loads): HBM reads of the 32768-point c2r fall from 1.9x to
+// 1.3x the input, -8% to -11% at 8192-32768 points on MI355X.
+MUFFT_HD constexpr int rv_default(int N, int kind = 0) {
+    return kind == 3 && N >= 8192 && N <= 32768 ? rv_with_chunked_loads(rv_default_base(N, kind))
+                                                : rv_default_base(N, kind);
 }
 
 // Radix sequence of NP passes: NP - 1 passes of radix R and one of the

@@ -37,6 +37,10 @@ constexpr int V16S_4 = rv_with_occ(V16S, 4);
 // 32 points per thread in radix-16 passes (lower register peak per butterfly).
 constexpr int V32ST_R16 = rv_with_radix(V32ST, 4);
 constexpr int V32PT_R16 = rv_with_radix(V32PT, 4);
+// c2r prologue loads in ordered chunks (the other kinds ignore the bit).
+constexpr int V32ST_C = rv_with_chunked_loads(V32ST);
+constexpr int V32PT_C = rv_with_chunked_loads(V32PT);
+constexpr int V16PT_C = rv_with_chunked_loads(V16PT);
 // Two workgroups per transform (split_impl.h), with the sub-transform's variant.
 constexpr int S2_16PT = rv_make_split(1, V16PT);
 constexpr int S2_32PT = rv_make_split(1, V32PT);
@@ -44,6 +48,8 @@ constexpr int S2_32ST = rv_make_split(1, V32ST);
 constexpr int S2_32ST_2 = rv_make_split(1, V32ST_2);
 constexpr int S2_16S_2 = rv_make_split(1, V16S_2);
 constexpr int S2_16ST_2 = rv_make_split(1, V16ST_2);
+constexpr int S2_32ST_C = rv_with_chunked_loads(S2_32ST);
+constexpr int S2_32ST_2C = rv_with_chunked_loads(S2_32ST_2);
 
 constexpr RowVariantEntry kRowVariants[] = {
     {1, V16P}, {2, V16P}, {3, V16P}, {4, V16P},
@@ -55,12 +61,13 @@ constexpr RowVariantEntry kRowVariants[] = {
     {10, V16P}, {10, V16PT}, {10, V8P}, {10, V16S}, {10, V16F}, {10, V32PT},
     {11, V16P}, {11, V16PT}, {11, V8P}, {11, V16S}, {11, V16F}, {11, V32PT},
     {12, V16P}, {12, V16PT}, {12, V8P}, {12, V16S}, {12, V16F}, {12, V32PT},
-    {13, V32P}, {13, V32PT}, {13, V16PT}, {13, V16S}, {13, V16S_4}, {13, V16PT_2},
+    {13, V32P}, {13, V32PT}, {13, V16PT}, {13, V16S}, {13, V16S_4}, {13, V16PT_2}, {13, V16PT_C},
     {14, V32P}, {14, V32PT}, {14, V16PT}, {14, V64PT}, {14, V32ST_2}, {14, V16S_2}, {14, V16ST_2},
-    {14, V32PT_R16},
+    {14, V32PT_R16}, {14, V32PT_C},
     {14, S2_16PT},
     {15, V32S}, {15, V32ST}, {15, V64ST}, {15, S2_32PT}, {15, S2_16PT}, {15, S2_32ST},
-    {15, S2_32ST_2}, {15, S2_16S_2}, {15, S2_16ST_2}, {15, V32ST_R16},
+    {15, S2_32ST_2}, {15, S2_16S_2}, {15, S2_16ST_2}, {15, V32ST_R16}, {15, V32ST_C},
+    {15, S2_32ST_C}, {15, S2_32ST_2C},
     {16, S2_32ST},
 };
 constexpr int kNumRowVariants = sizeof(kRowVariants) / sizeof(kRowVariants[0]);

@@ -94,8 +94,17 @@ __device__ __forceinline__ void row_transform(const float2 *X, const float2 *Y, 
     if constexpr (KIND == KIND_C2R) {
         // Pre-pass resolve (kernel.c:36-45), optionally after the fused multiply.
         // X[N - k] for k = lt + T*c is element (T - lt) + T*(E-1-c).
-        static_for<0, E>([&](auto C) {
-            constexpr int c = decltype(C)::value;
+        static_for<0, E>([&](auto J) {
+            // Chunked loads (rv_chunked_loads): point c of step j, chunks of
+            // CH points taken from the front and the back in turn; nothing
+            // of a chunk moves above its boundary, so its loads issue after
+            // the previous chunk's data arrived.
+            constexpr int j = decltype(J)::value;
+            constexpr int CH = 4;
+            constexpr bool CHUNKED = rv_chunked_loads(VAR) && E >= 2 * CH;
+            constexpr int q = j / CH, p = q / 2, r = j % CH;
+            constexpr int c = !CHUNKED ? j : (q % 2 == 0 ? p * CH + r : E - (p + 1) * CH + r);
+            if constexpr (CHUNKED && j > 0 && r == 0) __builtin_amdgcn_sched_barrier(0);
             const int k = lt + T * c;
             float2 z = make_float2(0.f, 0.f);
             if ((BUF || active) && !(zp && c >= E / 2)) {

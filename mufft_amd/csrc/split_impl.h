@@ -85,8 +85,16 @@ __device__ __forceinline__ void split_transform(const float2 *X, const float2 *Y
     }
 
     float2 v[E];
-    static_for<0, E>([&](auto C) {
-        constexpr int c = decltype(C)::value;
+    static_for<0, E>([&](auto J) {
+        // c2r chunked loads (rv_chunked_loads): as in rows_impl.h, points
+        // taken in chunks from the front and the back in turn, so the mirror
+        // thread's second read of each bin follows the first by one chunk.
+        constexpr int j = decltype(J)::value;
+        constexpr int CH = 4;
+        constexpr bool CHUNKED = KIND == KIND_C2R && rv_chunked_loads(VAR) && E >= 2 * CH;
+        constexpr int q = j / CH, p = q / 2, r = j % CH;
+        constexpr int c = !CHUNKED ? j : (q % 2 == 0 ? p * CH + r : E - (p + 1) * CH + r);
+        if constexpr (CHUNKED && j > 0 && r == 0) __builtin_amdgcn_sched_barrier(0);
         float2 acc = make_float2(0.f, 0.f);
         static_for<0, N1>([&](auto M1) {
             constexpr int n1 = decltype(M1)::value;
