@@ -110,6 +110,18 @@ __device__ __forceinline__ void stockham_passes(float2 (&v)[E], int lt, TwLoad t
     constexpr int NP = np_for(N, R);
     static_assert(R <= E && E % R == 0, "pass radix divides the points per thread");
     static_assert(!WAVE || T <= 64, "wave-local exchange needs the transform within one wave");
+    // Slot of point lt + T*c for the exchange reads.  The slot maps used here
+    // (padded: o + (o >> log2 E), scaled by a tile width and offset by a lane)
+    // split exactly over a multiple of E, so when E divides T every read is
+    // one per-thread base plus a compile-time offset; the compiler then keeps
+    // one address live instead of one per point, which it would otherwise
+    // hold (or spill) across all passes.
+    const int rd0 = slot(lt), rz = slot(0);
+    auto rd_slot = [&](auto C) {
+        constexpr int c = decltype(C)::value;
+        if constexpr (T % E == 0) return rd0 + (slot(T * c) - rz);
+        else return slot(lt + T * c);
+    };
     static_for<0, NP>([&](auto S) {
         constexpr int s = decltype(S)::value;
         constexpr int r = seq_radix(N, R, NP, s, REM_FIRST);
@@ -144,25 +156,16 @@ __device__ __forceinline__ void stockham_passes(float2 (&v)[E], int lt, TwLoad t
             if constexpr (XM != XM_SPLIT) {
                 publish([](float2 z) { return z; }, [&](int o, float2 z) { L[o] = z; });
                 exchange_sync<WAVE>();
-                static_for<0, E>([&](auto C) {
-                    constexpr int c = decltype(C)::value;
-                    v[c] = L[slot(lt + T * c)];
-                });
+                static_for<0, E>([&](auto C) { v[decltype(C)::value] = L[rd_slot(C)]; });
                 exchange_sync<WAVE>();
             } else {
                 publish([](float2 z) { return z.x; }, [&](int o, float x) { Lf[o] = x; });
                 exchange_sync<WAVE>();
-                static_for<0, E>([&](auto C) {
-                    constexpr int c = decltype(C)::value;
-                    v[c].x = Lf[slot(lt + T * c)];
-                });
+                static_for<0, E>([&](auto C) { v[decltype(C)::value].x = Lf[rd_slot(C)]; });
                 exchange_sync<WAVE>();
                 publish([](float2 z) { return z.y; }, [&](int o, float y) { Lf[o] = y; });
                 exchange_sync<WAVE>();
-                static_for<0, E>([&](auto C) {
-                    constexpr int c = decltype(C)::value;
-                    v[c].y = Lf[slot(lt + T * c)];
-                });
+                static_for<0, E>([&](auto C) { v[decltype(C)::value].y = Lf[rd_slot(C)]; });
                 exchange_sync<WAVE>();
             }
         }
