@@ -106,7 +106,7 @@ int mufft_hip_list_variants(unsigned N, int kind, int *out, int max) {
         set_err("no fused kernel for this size");
         return -1;
     }
-    return row_variant_list(l, out, max);
+    return row_variant_list(l, out, max, tk == TK_C2R);
 }
 
 int mufft_hip_default_variant(unsigned N, int kind) {
@@ -190,7 +190,7 @@ static int mufft_hip_autotune_1d_impl(unsigned N, int kind, unsigned howmany, do
         rc = -1;
     }
     int vars[64];
-    const int nv = rc ? 0 : std::min(row_variant_list(l, vars, 64), 64);
+    const int nv = rc ? 0 : std::min(row_variant_list(l, vars, 64, tk == TK_C2R), 64);
     std::vector<float2> h_out((size_t)odist * nval);
     for (int i = 0; i < nv && !rc; ++i) {
         Engine e;

@@ -17,9 +17,11 @@ namespace mufft {
 
 // Fused-kernel variant used inside the persistent kernel (1024 threads): 16
 // points per thread with split twiddles; at 2^CB_SPLIT_LOG2 points the same
-// variant runs each 16384-point half of the split transform.  It fits CB_LDS_BYTES.
+// variant runs each 16384-point half of the split transform.  It fits
+// CB_LDS_BYTES.  The c2r steps read each bin once through mirrored lanes
+// where a transform has whole waves (N >= 1024), in chunks below.
 MUFFT_HD constexpr int cb_variant(int N) {
-    return rv_with_chunked_loads(rv_make(16, 0, 1));   // the chunks apply to the c2r steps only
+    return N >= 1024 ? rv_with_mirror_lanes(rv_make(16, 0, 1)) : rv_with_chunked_loads(rv_make(16, 0, 1));
 }
 MUFFT_HD constexpr int cb_sub_n(int logn) { return logn >= CB_SPLIT_LOG2 ? (1 << (logn - 1)) : (1 << logn); }
 

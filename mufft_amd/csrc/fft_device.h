@@ -146,6 +146,23 @@ __device__ __forceinline__ void bstore_nt(__amdgpu_buffer_rsrc_t r, unsigned vof
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, (int)voff, (int)soff, 2);
 }
 
+// Thread numbering for the c2r pre-pass resolve, which pairs bin k with bin
+// N - k.  When T >= 64 threads hold points lt + T*c, the owners of k and
+// N - k are threads lt and T - lt; numbering thread t as mirror_lt<T>(t)
+// puts every such pair in lanes i and i ^ 32 of one wave.  Lanes 0 and 32 of
+// the first wave get 0 and T/2, which are their own partners.
+template <int T>
+__device__ __forceinline__ int mirror_lt(int t) {
+    static_assert(T >= 64 && (T & (T - 1)) == 0, "whole waves of one transform");
+    const int q = ((t >> 6) << 5) | (t & 31);
+    return (t & 32) == 0 ? q : (q == 0 ? T / 2 : T - q);
+}
+
+// The value v of lane i ^ 32 (the other half of the wave).
+__device__ __forceinline__ float2 swap_half_waves(float2 v) {
+    return make_float2(__shfl_xor(v.x, 32), __shfl_xor(v.y, 32));
+}
+
 // Padded LDS slot of point o when every E points get one spare slot.
 template <int LOGE>
 __device__ __forceinline__ int pad_slot(int o) { return o + (o >> LOGE); }

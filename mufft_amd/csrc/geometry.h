@@ -57,6 +57,12 @@ MUFFT_HD constexpr int ilog2(long long n) { return n <= 1 ? 0 : 1 + ilog2(n >> 1
 //              the line the first brought in (a whole transform's input, 256
 //              KB per CU at 32768 points, does not stay in a 4 MB L2 shared
 //              by 32 CUs for much longer).
+//   bit  29    c2r with T >= 64 threads per transform: every bin is read
+//              once.  Threads are numbered by mirror_lt (fft_device.h) so
+//              the owners of bins k and N - k are lanes i and i ^ 32 of one
+//              wave, which exchange the loaded bins in registers; the first
+//              Stockham pass runs under that numbering (bit 28 is then
+//              ignored).
 // The planner's static choice is rv_default(); the autotuner may pick any
 // registered variant.
 // ---------------------------------------------------------------------------
@@ -66,6 +72,10 @@ MUFFT_HD constexpr int rv_with_occ(int var, int wgs_per_cu) { return var | (wgs_
 MUFFT_HD constexpr int rv_with_radix(int var, int log2r) { return var | (log2r << 24); }
 MUFFT_HD constexpr int rv_with_chunked_loads(int var) { return var | (1 << 28); }
 MUFFT_HD constexpr bool rv_chunked_loads(int var) { return ((var >> 28) & 1) != 0; }
+MUFFT_HD constexpr int rv_with_mirror_lanes(int var) { return var | (1 << 29); }
+MUFFT_HD constexpr bool rv_mirror_lanes(int var) { return ((var >> 29) & 1) != 0; }
+// Bits 28-29 change only the c2r kernels.
+MUFFT_HD constexpr bool rv_c2r_only(int var) { return rv_chunked_loads(var) || rv_mirror_lanes(var); }
 MUFFT_HD constexpr int rv_split(int var) { return (var >> 16) & 0xf; }   // log2 N1
 MUFFT_HD constexpr int rv_occ(int var) { return (var >> 20) & 0xf; }
 MUFFT_HD constexpr int rv_sub(int var) { return var & ~(0xf << 16); }    // variant of the sub-transform
@@ -95,32 +105,30 @@ MUFFT_HD constexpr int rv_lds_per(int N, int var) {
 }
 // Static choice per size and transform kind (0/1 c2c, 2 r2c, 3 c2r; the real
 // transforms' fused size N is half the real length), from sweeps of all
-// registered variants on MI355X at 2^27 points per launch (1 GiB buffers, so
-// inputs come from HBM, not from the 256 MB Infinity Cache): split-product
-// twiddles pay off from 512 points on; c2r below 1024 points runs 8 points
-// per thread (-10% to -27%); c2c prefers 32 points per thread at 1024 and
-// 8192 while the real transforms, whose resolve adds registers, prefer 16 up
-// to 8192.  At 16384 c2c and r2c run 32 points per thread with the split
-// exchange and registers budgeted for two workgroups per CU (-20%, -7%), c2r
-// 32 points with the padded exchange.  At 32768 one workgroup per transform
-// (c2c and c2r 32 points, r2c 64 points per thread) beats the two-workgroup
-// split by 20-34% on cold data; the split only wins while a batch stays in
-// the Infinity Cache.  65536 runs on two workgroups per transform.
+// registered variants on MI355X, cold caches, at 2^25-2^27 points per launch:
+// split-product twiddles pay off from 512 points on; c2r below 1024 points
+// runs 8 points per thread (-10% to -27%), from 1024 to 8192 points 16 with
+// mirrored lanes (each bin read once, -5% to -10%); c2c and r2c run 16 points
+// per thread up to 8192 (c2c 1024 and 8192: -7%, -3% against 32).  At 16384
+// all kinds run 32 points per thread with the split exchange and registers
+// budgeted for two workgroups per CU (c2c -20%, r2c -7%; c2r with chunked
+// loads, -15%).  At 32768 c2c and r2c run one workgroup of 32 points per
+// thread per transform; the two-workgroup split reads its input twice and
+// is 20% slower at 2^27 points (faster only while a batch stays in the 256 MB
+// Infinity Cache); c2r runs the split with mirrored lanes, which reads each
+// bin once per workgroup (-3% to -6% against one workgroup).  65536 runs on
+// two workgroups per transform.
 MUFFT_HD constexpr int rv_default_base(int N, int kind) {
     return N < 512 ? (kind == 3 && N == 256 ? rv_make(8, 0, 0) : rv_make(16, 0, 0))
          : N == 512 ? (kind == 3 ? rv_make(8, 0, 0) : rv_make(16, 0, 1))
-         : N < 16384 ? ((N == 1024 || N == 8192) && kind < 2 ? rv_make(32, 0, 1) : rv_make(16, 0, 1))
-         : N == 16384 ? (kind == 3 ? rv_make(32, 0, 1) : rv_with_occ(rv_make(32, 1, 1), 2))
-         : N == 32768 ? (kind == 2 ? rv_make(64, 1, 1) : rv_make(32, 1, 1))
+         : N < 16384 ? (kind == 3 ? rv_with_mirror_lanes(rv_make(16, 0, 1)) : rv_make(16, 0, 1))
+         : N == 16384 ? (kind == 3 ? rv_with_chunked_loads(rv_with_occ(rv_make(32, 1, 1), 2))
+                                   : rv_with_occ(rv_make(32, 1, 1), 2))
+         : N == 32768 ? (kind == 3 ? rv_with_mirror_lanes(rv_with_occ(rv_make_split(1, rv_make(32, 1, 1)), 2))
+                                   : rv_make(32, 1, 1))
          : rv_make_split(1, rv_make(32, 1, 1));
 }
-// c2r from 8192 complex points on reads its bins in mirrored chunks
-// (rv_chunked_loads): HBM reads of the 32768-point c2r fall from 1.9x to
-// 1.3x the input, -8% to -11% at 8192-32768 points on MI355X.
-MUFFT_HD constexpr int rv_default(int N, int kind = 0) {
-    return kind == 3 && N >= 8192 && N <= 32768 ? rv_with_chunked_loads(rv_default_base(N, kind))
-                                                : rv_default_base(N, kind);
-}
+MUFFT_HD constexpr int rv_default(int N, int kind = 0) { return rv_default_base(N, kind); }
 
 // Radix sequence of NP passes: NP - 1 passes of radix R and one of the
 // remainder N / R^(NP-1), placed first (REM_FIRST) or last.  The twiddle

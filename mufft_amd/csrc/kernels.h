@@ -122,7 +122,9 @@ struct TwiddleArgs {
 // Fused row kernel of size 2^logN in variant `var` (geometry.h, row_variants.h).
 hipError_t launch_rows(int logN, int direction, int kind, int var, const RowArgs &a, hipStream_t s);
 // Registered variants of a size; returns the count (at most max are written).
-int row_variant_list(int logN, int *vars, int max);
+// Without c2r, the entries that differ from another only in c2r-specific
+// bits are left out (they run the same kernel for the other kinds).
+int row_variant_list(int logN, int *vars, int max, bool c2r = true);
 bool row_variant_registered(int logN, int var);
 // tx: tile width, col_TX(L) (0) or col_TX_alt(L) (plain mode only); ntiles must cover ncols.
 hipError_t launch_cols(int logL, int direction, const ColArgs &a, hipStream_t s, int mode = COL_PLAIN, int tx = 0);

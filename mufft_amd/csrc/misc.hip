@@ -33,10 +33,11 @@ int row_variant_index(int logN, int var) {
 }
 }  // namespace
 
-int row_variant_list(int logN, int *vars, int max) {
+int row_variant_list(int logN, int *vars, int max, bool c2r) {
     int n = 0;
     for (int i = 0; i < kNumRowVariants; ++i) {
         if (kRowVariants[i].logN != logN) continue;
+        if (!c2r && rv_c2r_only(kRowVariants[i].var)) continue;
         if (vars && n < max) vars[n] = kRowVariants[i].var;
         ++n;
     }

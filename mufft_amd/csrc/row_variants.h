@@ -50,6 +50,14 @@ constexpr int S2_16S_2 = rv_make_split(1, V16S_2);
 constexpr int S2_16ST_2 = rv_make_split(1, V16ST_2);
 constexpr int S2_32ST_C = rv_with_chunked_loads(S2_32ST);
 constexpr int S2_32ST_2C = rv_with_chunked_loads(S2_32ST_2);
+constexpr int V32ST_2C = rv_with_chunked_loads(V32ST_2);
+// c2r bins read once through mirrored lanes (the other kinds ignore the bit).
+constexpr int V16PT_M = rv_with_mirror_lanes(V16PT);
+constexpr int V32PT_M = rv_with_mirror_lanes(V32PT);
+constexpr int V32ST_M = rv_with_mirror_lanes(V32ST);
+constexpr int V32ST_2M = rv_with_mirror_lanes(V32ST_2);
+constexpr int S2_32ST_M = rv_with_mirror_lanes(S2_32ST);
+constexpr int S2_32ST_2M = rv_with_mirror_lanes(S2_32ST_2);
 
 constexpr RowVariantEntry kRowVariants[] = {
     {1, V16P}, {2, V16P}, {3, V16P}, {4, V16P},
@@ -58,16 +66,17 @@ constexpr RowVariantEntry kRowVariants[] = {
     {7, V16P}, {7, V8P},
     {8, V16P}, {8, V8P},
     {9, V16P}, {9, V16PT}, {9, V8P}, {9, V32PT},
-    {10, V16P}, {10, V16PT}, {10, V8P}, {10, V16S}, {10, V16F}, {10, V32PT},
-    {11, V16P}, {11, V16PT}, {11, V8P}, {11, V16S}, {11, V16F}, {11, V32PT},
-    {12, V16P}, {12, V16PT}, {12, V8P}, {12, V16S}, {12, V16F}, {12, V32PT},
+    {10, V16P}, {10, V16PT}, {10, V8P}, {10, V16S}, {10, V16F}, {10, V32PT}, {10, V16PT_M},
+    {11, V16P}, {11, V16PT}, {11, V8P}, {11, V16S}, {11, V16F}, {11, V32PT}, {11, V16PT_M},
+    {12, V16P}, {12, V16PT}, {12, V8P}, {12, V16S}, {12, V16F}, {12, V32PT}, {12, V16PT_M},
     {13, V32P}, {13, V32PT}, {13, V16PT}, {13, V16S}, {13, V16S_4}, {13, V16PT_2}, {13, V16PT_C},
+    {13, V16PT_M},
     {14, V32P}, {14, V32PT}, {14, V16PT}, {14, V64PT}, {14, V32ST_2}, {14, V16S_2}, {14, V16ST_2},
-    {14, V32PT_R16}, {14, V32PT_C},
+    {14, V32PT_R16}, {14, V32PT_C}, {14, V32ST_2C}, {14, V32PT_M}, {14, V32ST_2M},
     {14, S2_16PT},
     {15, V32S}, {15, V32ST}, {15, V64ST}, {15, S2_32PT}, {15, S2_16PT}, {15, S2_32ST},
     {15, S2_32ST_2}, {15, S2_16S_2}, {15, S2_16ST_2}, {15, V32ST_R16}, {15, V32ST_C},
-    {15, S2_32ST_C}, {15, S2_32ST_2C},
+    {15, S2_32ST_C}, {15, S2_32ST_2C}, {15, V32ST_M}, {15, S2_32ST_M}, {15, S2_32ST_2M},
     {16, S2_32ST},
 };
 constexpr int kNumRowVariants = sizeof(kRowVariants) / sizeof(kRowVariants[0]);

@@ -40,6 +40,8 @@ __device__ __forceinline__ void row_transform(const float2 *X, const float2 *Y, 
     // A transform of at most 64 threads lies in one wave (its threads are
     // consecutive): its exchanges need no workgroup barrier.
     constexpr bool WAVE = T <= 64;
+    // c2r with mirrored lanes: each bin read once (rv_mirror_lanes).
+    constexpr bool MIRROR = KIND == KIND_C2R && BUF && rv_mirror_lanes(VAR);
     auto slot = [](int o) { return XM == XM_FLAT ? o : pad_slot<LOGE>(o); };
 
     const bool zp = (flags & RF_ZERO_PAD) != 0;
@@ -88,10 +90,54 @@ __device__ __forceinline__ void row_transform(const float2 *X, const float2 *Y, 
     // entry per thread is read instead of one per point; W[N - k] = conj(W[k]).
     // (r2c reads W[lt] after the passes, where it is used.)
     float2 wb = make_float2(0.f, 0.f);
-    if constexpr (KIND == KIND_C2R) wb = ld_rtw(lt, 0);
+    int lm = lt;   // numbering of the loads and the first pass
+    if constexpr (MIRROR) lm = mirror_lt<T>(lt);
+    if constexpr (KIND == KIND_C2R) wb = ld_rtw(lm, 0);
 
     float2 v[E];
-    if constexpr (KIND == KIND_C2R) {
+    if constexpr (MIRROR) {
+        // Pre-pass resolve (kernel.c:36-45), each bin read once: this thread
+        // holds bins lm + T*c, and X[N - k] for k = lm + T*c is bin
+        // (T - lm) + T*(E-1-c) of the other half-wave's lane.  Lane 32
+        // (lm = T/2) holds its own partners; lane 0 (lm = 0) too, at
+        // X[N - T*c] = a[E - c], except bin N.  Selects, no branches: the
+        // prologue stays one block, so the loads issue ahead of their use.
+        auto bin = [&](int vp, int sp, bool edge) -> float2 {
+            float2 x = ld_in(vp, sp);
+            if (mul) {
+                float2 y = ld_in2(vp, sp);
+                if (cj) y = cconj(y);
+                x = cscale(cmul(x, y), norm);
+            }
+            // Adjoint of r2c (as below): interior bins halved, bins 0 and N real.
+            if (adj) x = edge ? make_float2(x.x, 0.f) : cscale(x, 0.5f);
+            return x;
+        };
+        const float2 xn = bin(0, N, true);
+        const bool l0 = lt == 0, lh = lt == 32;
+        float2 a[E];
+        auto finish = [&](auto C) {
+            constexpr int c = decltype(C)::value;
+            float2 b = swap_half_waves(a[E - 1 - c]);
+            b = lh ? a[E - 1 - c] : b;
+            b = l0 ? (c == 0 ? xn : a[(E - c) % E]) : b;
+            const float2 bc = cconj(b);
+            const float2 z = cadd(cadd(a[c], bc), cmul(csub(a[c], bc), twc<c, 2 * E, DIR>(wb)));
+            v[c] = (zp && c >= E / 2) ? make_float2(0.f, 0.f) : z;
+        };
+        // Points in groups {g, E-1-g}, closed under the partner map; lane 0
+        // also needs a[E-g] (group g-1) for point g and a[g+1] (group g+1)
+        // for point E-1-g, so point E-1-g is finished one group later.
+        static_for<0, E / 2 + 1>([&](auto G) {
+            constexpr int g = decltype(G)::value;
+            if constexpr (g < E / 2) {
+                a[g] = bin(lm, T * g, g == 0 && lm == 0);
+                a[E - 1 - g] = bin(lm, T * (E - 1 - g), false);
+                finish(G);
+            }
+            if constexpr (g > 0) finish(std::integral_constant<int, E - g>{});
+        });
+    } else if constexpr (KIND == KIND_C2R) {
         // Pre-pass resolve (kernel.c:36-45), optionally after the fused multiply.
         // X[N - k] for k = lt + T*c is element (T - lt) + T*(E-1-c).
         static_for<0, E>([&](auto J) {
@@ -154,7 +200,8 @@ __device__ __forceinline__ void row_transform(const float2 *X, const float2 *Y, 
 
     if constexpr (BUF) {
         const unsigned ntw = (unsigned)(N - seq_first(N, RAD, rv_NP(N, VAR), true));   // 0 for one pass
-        stockham_passes<N, E, DIR, XM, TWM, RAD, WAVE>(v, lt, TwBuffer{make_rsrc(tw, ntw * 8u)}, L, Lf, slot);
+        static_assert(!MIRROR || rv_NP(N, VAR) > 1, "the first pass hands its points over through LDS");
+        stockham_passes<N, E, DIR, XM, TWM, RAD, WAVE>(v, lt, TwBuffer{make_rsrc(tw, ntw * 8u)}, L, Lf, slot, lm);
     } else {
         stockham_passes<N, E, DIR, XM, TWM, RAD, WAVE>(v, lt, TwGlobal{tw}, L, Lf, slot);
     }

@@ -77,15 +77,75 @@ __device__ __forceinline__ void split_transform(const float2 *X, const float2 *Y
     //   split    w_NT^(k1*n) for k1 = 1: w_NT^(lt + T*c) = w_NT^lt * exp(2*pi*i*DIR*c/(2E)).
     // With N1 = 4 the split twiddle's rotation depends on k1, so it is read per point.
     // (r2c reads its base W[2*lt + k1] after the passes, where it is used.)
+    // c2r with mirrored lanes (rv_mirror_lanes): loads and the first pass
+    // run under the numbering lm = mirror_lt(lt).
+    constexpr bool MIRROR = KIND == KIND_C2R && rv_mirror_lanes(VAR) && T >= 64;
+    int lm = lt;
+    if constexpr (MIRROR) lm = mirror_lt<T>(lt);
     float2 wb = make_float2(0.f, 0.f);
-    if constexpr (KIND == KIND_C2R) wb = ld(rrtw, lt, 0);
+    if constexpr (KIND == KIND_C2R) wb = ld(rrtw, lm, 0);
     float2 sb = make_float2(0.f, 0.f);
     if constexpr (N1 == 2) {
-        if (k1 > 0) sb = ld(rstw, lt, 0);
+        if (k1 > 0) sb = ld(rstw, lm, 0);
     }
 
     float2 v[E];
-    static_for<0, E>([&](auto J) {
+    if constexpr (MIRROR) {
+        // Each bin read once.  Point (c, n1) = lm + T*c + N2*n1; X[NT - m]
+        // is point (E-1-c, 1-n1) of the other half-wave's lane.  Lane 32
+        // (lm = T/2) holds its own partners; lane 0 (lm = 0) holds them at
+        // (E-c, 1-n1), except X[NT] for (0, 0) and itself for (0, 1).
+        auto bin = [&](int sp, bool edge) -> float2 {
+            float2 x = ld(rin, lm, sp);
+            if (mul) {
+                float2 y = ld(rin2, lm, sp);
+                if (cj) y = cconj(y);
+                x = cscale(cmul(x, y), norm);
+            }
+            if (adj) x = edge ? make_float2(x.x, 0.f) : cscale(x, 0.5f);
+            return x;
+        };
+        float2 xn = ld(rin, 0, NT);   // bin NT, for lane 0
+        if (mul) {
+            float2 y = ld(rin2, 0, NT);
+            if (cj) y = cconj(y);
+            xn = cscale(cmul(xn, y), norm);
+        }
+        if (adj) xn = make_float2(xn.x, 0.f);
+        const bool l0 = lt == 0, lh = lt == 32;
+        float2 x0[E], x1[E];
+        // v[c] from its own points (c, 0), (c, 1) and their partners.
+        auto finish = [&](auto C) {
+            constexpr int c = decltype(C)::value;
+            float2 p0 = swap_half_waves(x1[E - 1 - c]);
+            float2 p1 = swap_half_waves(x0[E - 1 - c]);
+            p0 = lh ? x1[E - 1 - c] : p0;
+            p1 = lh ? x0[E - 1 - c] : p1;
+            p0 = l0 ? (c == 0 ? xn : x1[(E - c) % E]) : p0;
+            p1 = l0 ? (c == 0 ? x1[0] : x0[(E - c) % E]) : p1;
+            // W[lm + T*c + N2*n1] = W[lm] * exp(2*pi*i*DIR*(c + E*n1)/(4E)).
+            const float2 b0 = cconj(p0), b1 = cconj(p1);
+            const float2 z0 = cadd(cadd(x0[c], b0), cmul(csub(x0[c], b0), twc<c, 4 * E, DIR>(wb)));
+            const float2 z1 = cadd(cadd(x1[c], b1), cmul(csub(x1[c], b1), twc<c + E, 4 * E, DIR>(wb)));
+            float2 acc = zp ? z0 : cadd(z0, cmul(z1, rot(1)));
+            if (k1 > 0) acc = cmul(acc, twc<c, 2 * E, DIR>(sb));
+            v[c] = acc;
+        };
+        // Points in groups {g, E-1-g}, closed under the partner map; lane 0
+        // also needs group g-1's E-g for point g and group g+1's g+1 for point
+        // E-1-g, so point E-1-g is finished one group later.
+        static_for<0, E / 2 + 1>([&](auto G) {
+            constexpr int g = decltype(G)::value;
+            if constexpr (g < E / 2) {
+                x0[g] = bin(T * g, g == 0 && lm == 0);
+                x1[g] = bin(T * g + N2, false);
+                x0[E - 1 - g] = bin(T * (E - 1 - g), false);
+                x1[E - 1 - g] = bin(T * (E - 1 - g) + N2, false);
+                finish(G);
+            }
+            if constexpr (g > 0) finish(std::integral_constant<int, E - g>{});
+        });
+    } else static_for<0, E>([&](auto J) {
         // c2r chunked loads (rv_chunked_loads): as in rows_impl.h, points
         // taken in chunks from the front and the back in turn, so the mirror
         // thread's second read of each bin follows the first by one chunk.
@@ -142,8 +202,9 @@ __device__ __forceinline__ void split_transform(const float2 *X, const float2 *Y
     });
 
     constexpr int RAD = rv_R(N2, VAR);
+    static_assert(!MIRROR || np_for(N2, RAD) > 1, "the first pass hands its points over through LDS");
     stockham_passes<N2, E, DIR, XM, TWM, RAD>(v, lt, TwBuffer{make_rsrc(tw, (N2 - seq_first(N2, RAD, np_for(N2, RAD), true)) * 8u)}, L, Lf,
-                                              slot);
+                                              slot, lm);
 
     // v[c] now holds X[N1*k2 + k1] for k2 = lt + T*c.
     if constexpr (KIND != KIND_R2C) {

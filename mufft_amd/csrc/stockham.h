@@ -101,13 +101,18 @@ __device__ __forceinline__ void exchange_sync() {
 // butterflies of a pass, butterfly j being b = lt + T*j over its points
 // v[j + m*J].  `twl` reads the twiddle table of that sequence.  `Slot(o)`
 // maps a point index to its LDS slot; `L` is the complex LDS view, `Lf` the
-// float view used by the split exchange.
+// float view used by the split exchange.  The first pass may hold its points
+// under another thread numbering: with lt0 >= 0, v[c] holds point lt0 + T*c
+// on entry (lt0 a permutation of the threads); the first pass has no
+// twiddles and its results go through LDS, so the later passes and the
+// output use lt.
 template <int N, int E, int DIR, int XM, int TWM, int R = E, bool WAVE = false, bool REM_FIRST = true,
           typename TwLoad, typename Slot>
 __device__ __forceinline__ void stockham_passes(float2 (&v)[E], int lt, TwLoad twl,
-                                                float2 *L, float *Lf, Slot slot) {
+                                                float2 *L, float *Lf, Slot slot, int lt0 = -1) {
     constexpr int T = N / E;
     constexpr int NP = np_for(N, R);
+    if (lt0 < 0) lt0 = lt;
     static_assert(R <= E && E % R == 0, "pass radix divides the points per thread");
     static_assert(!WAVE || T <= 64, "wave-local exchange needs the transform within one wave");
     // Slot of point lt + T*c for the exchange reads.  The slot maps used here
@@ -144,7 +149,7 @@ __device__ __forceinline__ void stockham_passes(float2 (&v)[E], int lt, TwLoad t
             auto publish = [&](auto get, auto put) {
                 static_for<0, J>([&](auto JJ) {
                     constexpr int j = decltype(JJ)::value;
-                    const int b = lt + T * j;
+                    const int b = (s == 0 ? lt0 : lt) + T * j;
                     const int k = b & (p - 1);
                     const int base = (b - k) * r + k;
                     static_for<0, r>([&](auto Q) {

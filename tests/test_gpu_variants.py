@@ -102,3 +102,54 @@ def test_autotune_selects_valid_variant(gpu, tmp_path, N, kind):
         assert rel_rms(_run(gpu, kind, N, x, batch), ref) < 2e-6
     finally:
         L.mufft_hip_set_variant(N, kind, 0)
+
+
+def _inner(a, b):
+    a = np.asarray(a, np.complex128 if np.iscomplexobj(a) else np.float64)
+    b = np.asarray(b, np.complex128 if np.iscomplexobj(b) else np.float64)
+    return float(np.real(np.sum(a * np.conj(b))))
+
+
+@pytest.mark.parametrize("logn", range(9, 16))
+def test_c2r_variants_with_fused_prologue_flags(gpu, logn):
+    """The c2r prologue of every variant under its runtime flags: the fused
+    multiply with a conjugated second operand and a norm (the convolution's
+    inverse), and the adjoint weighting of an r2c plan (plain and zero-padded),
+    checked by FP64 and by the dot-product identity.  The variants read the
+    bins in different orders and lane layouts (chunked, mirrored lanes), so
+    each one is checked here with every flag."""
+    import mufft_amd as M
+    import torch
+    from mufft_amd import _lib
+    L = _lib.load()
+    N = 1 << (logn + 1)
+    nb = N // 2 + 1
+    B = 3
+    X = cplx(logn * 19 + 1, B * nb, 0.5).reshape(B, nb)
+    Y = cplx(logn * 19 + 2, B * nb, 0.5).reshape(B, nb)
+    ref_mul = fp64.c2r(0.25 * X.astype(np.complex128) * np.conj(Y), N)
+    try:
+        for var in _variants(N, 3):
+            assert L.mufft_hip_set_variant(N, 3, var) == 0
+            p = M.Plan1D("c2r", N)
+            out = empty_dev((B, N), np.float32, gpu)
+            p.execute_mul_batched(out, to_dev(X, gpu), to_dev(Y, gpu), B, idist2=nb, norm=0.25, conj2=True)
+            torch.cuda.synchronize()
+            y = from_dev(out)
+            assert rel_rms(y, ref_mul) < 2e-6, (N, hex(var))
+            p.close()
+            for fl, nin in ((0, N), (M.FLAG_ZERO_PAD_UPPER_HALF, N // 2)):
+                q = M.Plan1D("r2c", N, M.FORWARD, fl)
+                x = real(logn * 19 + 3, B * nin).reshape(B, nin)
+                g = cplx(logn * 19 + 4, B * nb).reshape(B, nb)
+                Tx = empty_dev((B, nb), np.complex64, gpu)
+                Tg = empty_dev((B, N), np.float32, gpu)
+                q.execute_batched(Tx, to_dev(x, gpu), B)
+                q.execute_adjoint_batched(Tg, to_dev(g, gpu), B)
+                torch.cuda.synchronize()
+                tx, tg = from_dev(Tx), from_dev(Tg)[:, :nin]
+                lhs, rhs = _inner(tx, g), _inner(x, tg)
+                assert abs(lhs - rhs) <= 2e-6 * np.sqrt(_inner(tx, tx) * _inner(g, g)), (N, hex(var), fl)
+                q.close()
+    finally:
+        L.mufft_hip_set_variant(N, 3, 0)

@@ -47,17 +47,26 @@ def test_every_size_has_variants_and_a_registered_default(lib, kind):
 
 
 def test_defaults_follow_the_measured_policy(lib):
-    # 16 points per thread for the real transforms at 1024 complex points, 32 for c2c;
-    # c2r 8 points below 1024; c2c 16384 with registers for two workgroups per CU;
-    # 32768 complex points in one workgroup (r2c 64 points per thread); 65536 split
+    # 16 points per thread up to 8192 complex points (c2r: 8 below 1024, mirrored
+    # lanes from 1024); 16384 with registers for two workgroups per CU (c2r with
+    # chunked loads); 32768 complex points in one workgroup of 32 points per
+    # thread, but c2r split over two workgroups with mirrored lanes; 65536 split
     # over two workgroups.
-    assert lib.mufft_hip_default_variant(1024, 0) & 0xFF == 32
-    assert lib.mufft_hip_default_variant(8192, 0) & 0xFF == 32
+    M = 1 << 29
+    assert lib.mufft_hip_default_variant(1024, 0) & 0xFF == 16
+    assert lib.mufft_hip_default_variant(8192, 0) & 0xFF == 16
     assert lib.mufft_hip_default_variant(2048, 2) & 0xFF == 16
     assert lib.mufft_hip_default_variant(1024, 3) & 0xFF == 8
+    assert lib.mufft_hip_default_variant(2048, 3) & M
+    assert lib.mufft_hip_default_variant(16384, 3) & M
+    assert not lib.mufft_hip_default_variant(16384, 2) & M
     assert (lib.mufft_hip_default_variant(16384, 0) >> 20) & 0xF == 2
+    assert (lib.mufft_hip_default_variant(32768, 3) >> 20) & 0xF == 2
     assert (lib.mufft_hip_default_variant(32768, 0) >> 16) & 0xF == 0
-    assert lib.mufft_hip_default_variant(65536, 2) & 0xFF == 64
+    assert lib.mufft_hip_default_variant(65536, 2) & 0xFF == 32
+    assert (lib.mufft_hip_default_variant(65536, 2) >> 16) & 0xF == 0
+    v = lib.mufft_hip_default_variant(65536, 3)
+    assert (v >> 16) & 0xF == 1 and v & M
     assert (lib.mufft_hip_default_variant(65536, 0) >> 16) & 0xF == 1
 
 
