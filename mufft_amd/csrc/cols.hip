@@ -41,11 +41,24 @@ __global__ void __launch_bounds__(col_T(L_) * (TXP ? TXP : col_TX(L_))) fft_cols
     }
 
     float2 v[E];
-    const float2 *src = a.in + ib + col;
-    static_for<0, E>([&](auto C) {
-        constexpr int c = decltype(C)::value;
-        v[c] = active ? src[(long long)(t + T * c) * a.in_rs] : make_float2(0.f, 0.f);
-    });
+    if constexpr (MODE == COL_STREAM) {
+        // Image bytes below 2^32 (launch_cols checks): one per-lane offset,
+        // the row offset of each point in a scalar register.
+        const unsigned span = (unsigned)(((long long)(L_ - 1) * a.in_rs + a.ncols) * 8);
+        const __amdgpu_buffer_rsrc_t rin = make_rsrc(a.in + ib, span);
+        const unsigned vo = (unsigned)((long long)t * a.in_rs + col) * 8u;
+        static_for<0, E>([&](auto C) {
+            constexpr int c = decltype(C)::value;
+            const unsigned so = (unsigned)__builtin_amdgcn_readfirstlane((int)((long long)(T * c) * a.in_rs)) * 8u;
+            v[c] = active ? bload_nt(rin, vo, so) : make_float2(0.f, 0.f);
+        });
+    } else {
+        const float2 *src = a.in + ib + col;
+        static_for<0, E>([&](auto C) {
+            constexpr int c = decltype(C)::value;
+            v[c] = active ? src[(long long)(t + T * c) * a.in_rs] : make_float2(0.f, 0.f);
+        });
+    }
 
     // Remainder pass last (seq_radix): the lanes of one column thread share its twiddles.
     stockham_passes<L_, E, DIR, XM, col_twm(L_), E, false, false>(v, t, TwGlobal{a.tw}, smem,
@@ -61,11 +74,22 @@ __global__ void __launch_bounds__(col_T(L_) * (TXP ? TXP : col_TX(L_))) fft_cols
                 v[c] = cmul(v[c], cmul(a.coarse[e >> a.log2s], a.fine[e & smask]));
             });
         }
-        float2 *dst = a.out + ob + col;
-        static_for<0, E>([&](auto C) {
-            constexpr int c = decltype(C)::value;
-            dst[(long long)(t + T * c) * a.out_rs] = v[c];
-        });
+        if constexpr (MODE == COL_STREAM) {
+            const unsigned span = (unsigned)(((long long)(L_ - 1) * a.out_rs + a.ncols) * 8);
+            const __amdgpu_buffer_rsrc_t rout = make_rsrc(a.out + ob, span);
+            const unsigned vo = (unsigned)((long long)t * a.out_rs + col) * 8u;
+            static_for<0, E>([&](auto C) {
+                constexpr int c = decltype(C)::value;
+                const unsigned so = (unsigned)__builtin_amdgcn_readfirstlane((int)((long long)(T * c) * a.out_rs)) * 8u;
+                bstore_nt(rout, vo, so, v[c]);
+            });
+        } else {
+            float2 *dst = a.out + ob + col;
+            static_for<0, E>([&](auto C) {
+                constexpr int c = decltype(C)::value;
+                dst[(long long)(t + T * c) * a.out_rs] = v[c];
+            });
+        }
     }
 }
 
@@ -211,6 +235,12 @@ hipError_t launch_cols(int logL, int direction, const ColArgs &a, hipStream_t s,
         k = direction < 0 ? col_mode_kernel_ptr<-1, COL_TWIDDLE>(logL) : col_mode_kernel_ptr<1, COL_TWIDDLE>(logL);
     else if (mode == COL_TWO_LEVEL)
         k = direction < 0 ? col_mode_kernel_ptr<-1, COL_TWO_LEVEL>(logL) : col_mode_kernel_ptr<1, COL_TWO_LEVEL>(logL);
+    else if (mode == COL_STREAM) {
+        // Buffer offsets are 32-bit: the image must span less than 4 GiB.
+        const long long rs = a.in_rs > a.out_rs ? a.in_rs : a.out_rs;
+        if (((long long)(L - 1) * rs + a.ncols) * 8 >= (1LL << 32) || a.ncols > rs) return hipErrorInvalidValue;
+        k = direction < 0 ? col_mode_kernel_ptr<-1, COL_STREAM>(logL) : col_mode_kernel_ptr<1, COL_STREAM>(logL);
+    }
     if (!k) return hipErrorInvalidValue;
     if (mode == COL_TWIDDLE && (!a.coarse || !a.fine)) return hipErrorInvalidValue;
     if (mode == COL_TWO_LEVEL && a.img_div <= 0) return hipErrorInvalidValue;

@@ -119,7 +119,11 @@ int run_large(const Engine &e, const float2 *src, long long sd, float2 *dst, lon
     c.ncols = n1;
     c.ntiles = (n1 + col_TX(n2) - 1) / col_TX(n2);
     c.batch = batch;
-    HIP_TRY(launch_cols(e.log_n2, e.direction, c, st));
+    // Columns of 64-1024 points (tile rows of whole cache lines) stream:
+    // -3% to -7% at 2^17-2^22 points on MI355X.  With narrower tile rows
+    // (2048 points and up) neighbouring tiles share lines and streaming
+    // accesses double the time.
+    HIP_TRY(launch_cols(e.log_n2, e.direction, c, st, e.log_n2 >= 6 && e.log_n2 <= 10 ? COL_STREAM : COL_PLAIN));
     return 0;
 }
 

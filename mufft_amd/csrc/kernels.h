@@ -43,9 +43,12 @@ struct ColArgs {
     long long in_bs2, out_bs2;
 };
 
-// Column-kernel modes (the plain one serves the 2D and four-step passes; the
-// other two are the middle and last passes of the three-pass transform).
-enum ColMode : int { COL_PLAIN = 0, COL_TWIDDLE = 1, COL_TWO_LEVEL = 2 };
+// Column-kernel modes (the plain one serves the 2D and four-step passes;
+// COL_TWIDDLE and COL_TWO_LEVEL are the middle and last passes of the
+// three-pass transform; COL_STREAM is the plain transform with streaming
+// (non-temporal) buffer accesses, for the two-pass four-step's second launch
+// on columns of 64 to 1024 points, whose tile rows are whole cache lines).
+enum ColMode : int { COL_PLAIN = 0, COL_TWIDDLE = 1, COL_TWO_LEVEL = 2, COL_STREAM = 3 };
 
 // First half of a four-step transform of N = N1 * N2 points (n = n1*N2 + n2):
 // the N1-point transforms down the columns of the N1 x N2 matrix, times

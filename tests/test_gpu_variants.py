@@ -153,3 +153,51 @@ def test_c2r_variants_with_fused_prologue_flags(gpu, logn):
                 q.close()
     finally:
         L.mufft_hip_set_variant(N, 3, 0)
+
+
+@pytest.mark.parametrize("logn", range(9, 16))
+def test_r2c_variants_with_epilogue_flags(gpu, logn):
+    """The r2c epilogue of every variant under its runtime flags: the full
+    spectrum (bins N..2N-1 too), zero-padded input, and the adjoint of a c2r
+    plan (checked by the dot-product identity).  The variants pair bins k and
+    N - k differently (through LDS, or in registers between mirrored lanes)."""
+    import mufft_amd as M
+    import torch
+    from mufft_amd import _lib
+    L = _lib.load()
+    N = 1 << (logn + 1)
+    nb = N // 2 + 1
+    B = 3
+    x = real(logn * 23 + 1, B * N, 0.5).reshape(B, N)
+    full_ref = np.fft.fft(x.astype(np.float64), axis=-1)
+    xz = x[:, : N // 2]
+    zp_ref = fp64.r2c(np.concatenate([xz, np.zeros_like(xz)], axis=1))
+    try:
+        for var in _variants(N, 2):
+            assert L.mufft_hip_set_variant(N, 2, var) == 0
+            p = M.Plan1D("r2c", N, M.FORWARD, M.FLAG_FULL_R2C)
+            out = empty_dev((B, N), np.complex64, gpu)
+            p.execute_batched(out, to_dev(x, gpu), B)
+            torch.cuda.synchronize()
+            assert rel_rms(from_dev(out), full_ref) < 2e-6, (N, hex(var), "full")
+            p.close()
+            p = M.Plan1D("r2c", N, M.FORWARD, M.FLAG_ZERO_PAD_UPPER_HALF)
+            out = empty_dev((B, nb), np.complex64, gpu)
+            p.execute_batched(out, to_dev(np.ascontiguousarray(xz), gpu), B)
+            torch.cuda.synchronize()
+            assert rel_rms(from_dev(out), zp_ref) < 2e-6, (N, hex(var), "zero pad")
+            p.close()
+            q = M.Plan1D("c2r", N)
+            X = cplx(logn * 23 + 2, B * nb).reshape(B, nb)
+            g = real(logn * 23 + 3, B * N).reshape(B, N)
+            Tx = empty_dev((B, N), np.float32, gpu)
+            Tg = empty_dev((B, nb), np.complex64, gpu)
+            q.execute_batched(Tx, to_dev(X, gpu), B)
+            q.execute_adjoint_batched(Tg, to_dev(g, gpu), B)
+            torch.cuda.synchronize()
+            tx, tg = from_dev(Tx), from_dev(Tg)
+            lhs, rhs = _inner(tx, g), _inner(X, tg)
+            assert abs(lhs - rhs) <= 2e-6 * np.sqrt(_inner(tx, tx) * _inner(g, g)), (N, hex(var), "adjoint")
+            q.close()
+    finally:
+        L.mufft_hip_set_variant(N, 2, 0)
