@@ -577,11 +577,12 @@ def main():
         return (time.perf_counter() - t) * 1e3 / reps
 
     # candidates: the tuned variant, the static default and one workgroup of
-    # 32 points per thread (V32ST), which won the step on some boxes
-    one_wg = 0x1120
+    # 32 points per thread (V32ST; for c2r with chunked loads), which won the
+    # step on some boxes
+    one_wg_r, one_wg_c = 0x1120, 0x10001120
     def_r, def_c = L.mufft_hip_default_variant(REAL_N, 2), L.mufft_hip_default_variant(REAL_N, 3)
-    cand_r = sorted({int(tuned[f"r2c_{REAL_N}"]["variant"], 16), def_r, one_wg})
-    cand_c = sorted({int(tuned[f"c2r_{REAL_N}"]["variant"], 16), def_c, one_wg})
+    cand_r = sorted({int(tuned[f"r2c_{REAL_N}"]["variant"], 16), def_r, one_wg_r})
+    cand_c = sorted({int(tuned[f"c2r_{REAL_N}"]["variant"], 16), def_c, one_wg_c})
     pairs = {(rv, cv): make_pair(rv, cv) for rv in cand_r for cv in cand_c}
     rounds = {k: [] for k in pairs}
     for _ in range(3):
