@@ -188,3 +188,38 @@ def test_roundtrip_batched_4096(gpu):
     torch.cuda.synchronize()
     np.testing.assert_array_equal(from_dev(one).view(np.uint32), from_dev(dX[17]).view(np.uint32))
     assert math.isfinite(float(np.abs(from_dev(dX)).max()))
+
+
+@pytest.mark.parametrize("kind", ["r2c", "c2r"])
+def test_65536_real_small_and_large_launches(gpu, kind):
+    """While no variant is chosen, a 65536-point real plan runs its
+    two-workgroup split for launches of at most two transforms per CU and one
+    workgroup per transform above (rv_default_small).  Both launches against
+    FP64 on the same leading transforms (test.c:58 bound), and the two results
+    within rounding of each other."""
+    import mufft_amd as M
+    import torch
+    from mufft_amd import _lib
+    L = _lib.load()
+    N = 65536
+    tk = 2 if kind == "r2c" else 3
+    assert L.mufft_hip_get_variant(N, tk) == L.mufft_hip_default_variant(N, tk)   # nothing chosen
+    big = 2 * torch.cuda.get_device_properties(0).multi_processor_count + 8
+    p = M.Plan1D(kind, N)
+    outs = []
+    for B in (3, big):
+        if kind == "r2c":
+            x = real(0x6501, B * N, 0.5).reshape(B, N)
+            y = empty_dev((B, N // 2 + 1), np.complex64, gpu)
+            ref = fp64.r2c(x[:3])
+        else:
+            x = cplx(0x6502, B * (N // 2 + 1), 0.5).reshape(B, N // 2 + 1)
+            y = empty_dev((B, N), np.float32, gpu)
+            ref = fp64.c2r(x[:3], N)
+        p.execute_batched(y, to_dev(x, gpu), B)
+        torch.cuda.synchronize()
+        got = from_dev(y)[:3]
+        assert np.max(np.abs(got - ref)) < abs_tol(N), (kind, B)
+        outs.append(got)
+    assert rel_rms(outs[1], outs[0]) < 1e-6
+    p.close()
