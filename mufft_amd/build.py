@@ -17,6 +17,7 @@ LIB = PKG / "libmufft_hip.so"
 # Kernel units first (the row-kernel instantiations take the longest), then
 # the host units (planner, executor and C ABI).
 SOURCES = [
+    "rows_c2r_large.hip", "rows_r2c_large.hip", "rows_c2c_fwd_large.hip", "rows_c2c_inv_large.hip",
     "rows_c2c_fwd.hip", "rows_c2c_inv.hip", "rows_r2c.hip", "rows_c2r.hip", "conv_batch.hip",
     "cols.hip", "misc.hip",
     "host_common.hip", "exec.hip", "plan_1d.hip", "plan_2d.hip", "plan_conv.hip", "runtime.hip",
@@ -62,9 +63,7 @@ def build(jobs: int | None = None) -> Path:
 # and UndefinedBehaviorSanitizer on the host side only (GPU code is compiled
 # unchanged; GPU sanitizers are not used) and linked with the kernel units of
 # the regular build into a separate library that the CPU tests load.
-HOST_UNITS = [s for s in SOURCES if s not in (
-    "rows_c2c_fwd.hip", "rows_c2c_inv.hip", "rows_r2c.hip", "rows_c2r.hip", "conv_batch.hip", "cols.hip",
-    "misc.hip")]
+HOST_UNITS = [s for s in SOURCES if not s.startswith("rows_") and s not in ("conv_batch.hip", "cols.hip", "misc.hip")]
 SAN_LIB = BUILD / "libmufft_hip_san.so"
 SAN_FLAGS = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
              "-Xarch_host", "-fno-sanitize-recover=undefined", "-Xarch_host", "-fno-omit-frame-pointer",

@@ -30,6 +30,9 @@ enum RowFlag : unsigned {
     // stores, for outputs that leave the device's caches anyway).
     RF_KEEP_OUT = 1u << 6,
 };
+// Every flag: a launch with none of them runs the kernels' plain
+// instantiation, where the flags are compile-time zero.
+constexpr unsigned RF_ALL = (1u << 7) - 1u;
 
 MUFFT_HD constexpr int ilog2(long long n) { return n <= 1 ? 0 : 1 + ilog2(n >> 1); }
 

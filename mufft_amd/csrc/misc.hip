@@ -7,10 +7,16 @@
 
 namespace mufft {
 
+// Each fills its (direction, kind) table: entries [0, kRowSplitIndex), and
+// the _large ones [kRowSplitIndex, kNumRowVariants).
 void row_kernels_c2c_fwd(const void **out);
 void row_kernels_c2c_inv(const void **out);
 void row_kernels_r2c(const void **out);
 void row_kernels_c2r(const void **out);
+void row_kernels_c2c_fwd_large(const void **out);
+void row_kernels_c2c_inv_large(const void **out);
+void row_kernels_r2c_large(const void **out);
+void row_kernels_c2r_large(const void **out);
 
 namespace {
 struct RowTables {
@@ -20,6 +26,10 @@ struct RowTables {
         row_kernels_c2c_inv(t[1]);
         row_kernels_r2c(t[2]);
         row_kernels_c2r(t[3]);
+        row_kernels_c2c_fwd_large(t[0]);
+        row_kernels_c2c_inv_large(t[1]);
+        row_kernels_r2c_large(t[2]);
+        row_kernels_c2r_large(t[3]);
     }
 };
 const RowTables &row_tables() {

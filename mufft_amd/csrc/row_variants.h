@@ -81,4 +81,11 @@ constexpr RowVariantEntry kRowVariants[] = {
 };
 constexpr int kNumRowVariants = sizeof(kRowVariants) / sizeof(kRowVariants[0]);
 
+// The instantiation units split the registry here (the entries are ordered
+// by size; from 32768 points on they take most of the compile time).
+constexpr int row_split_index(int i = 0) {
+    return i >= kNumRowVariants || kRowVariants[i].logN >= 15 ? i : row_split_index(i + 1);
+}
+constexpr int kRowSplitIndex = row_split_index();
+
 }  // namespace mufft

@@ -1,9 +1,9 @@
-// Instantiations of the fused row kernel: c2r (direction 1), every
-// registered variant (row_variants.h).
+// Instantiations of the fused row kernel: c2r (direction 1), the registered
+// variants (row_variants.h) below 32768 points.
 #include "rows_impl.h"
 
 namespace mufft {
 
-void row_kernels_c2r(const void **out) { fill_row_kernels<1, KIND_C2R>(out); }
+void row_kernels_c2r(const void **out) { fill_row_kernels<1, KIND_C2R, 0, kRowSplitIndex>(out); }
 
 }  // namespace mufft
