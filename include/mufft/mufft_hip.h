@@ -106,9 +106,7 @@ int mufft_hip_mul_reduce(void *output, const void *a, const void *b, unsigned n,
  * process-wide choice.  No counterpart in the reference, whose plans always
  * take its static decomposition (fft.c:325-379). */
 mufft_plan_1d *mufft_hip_create_plan_1d_variant(unsigned N, int kind, unsigned flags, int variant);
-/* The fused-kernel variant a plan runs (0 when it runs no fused row kernel;
- * a plan on the static choice may run another variant for launches of at
- * most two transforms per CU, see mufft_hip_default_variant). */
+/* The fused-kernel variant a plan runs (0 when it runs no fused row kernel). */
 int mufft_hip_plan_1d_variant(const mufft_plan_1d *plan);
 
 /* Number of kernel launches one execution of the plan issues. */
@@ -136,10 +134,7 @@ int mufft_hip_twiddle_table_mode(int mode, unsigned n, unsigned param, int direc
  * variant fixes the points per thread, the LDS exchange mode and the twiddle
  * scheme.  kind: 0 c2c forward, 1 c2c inverse, 2 r2c, 3 c2r; N is the size
  * passed to the plan creator (real length for r2c / c2r).  A choice applies
- * to plans created afterwards, process-wide.  While a size has no choice,
- * its plans run the static default, except that the 65536-point real
- * transforms run their two-workgroup split variant for launches of at most
- * two transforms per CU (512 on MI355X). */
+ * to plans created afterwards, process-wide. */
 int mufft_hip_list_variants(unsigned N, int kind, int *out, int max);
 int mufft_hip_default_variant(unsigned N, int kind);
 int mufft_hip_get_variant(unsigned N, int kind);

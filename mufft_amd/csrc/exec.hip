@@ -137,21 +137,20 @@ int run_rows(const Engine &e, int kind, const float2 *rtw, const float2 *in, lon
              unsigned flags, float norm, DevBuf &s0, DevBuf &s1, hipStream_t st) {
     if (batch <= 0) return 0;
     if (e.row_fused()) {
-        const bool small = e.row_var_small != 0 && batch <= e.small_max;
         RowArgs a{};
         a.in = in;
         a.in2 = in2;
         a.out = out;
-        a.tw = small ? e.tw_row_small : e.tw_row;
+        a.tw = e.tw_row;
         a.rtw = rtw;
-        a.stw = small ? e.tw_split_small : e.tw_split;
+        a.stw = e.tw_split;
         a.idist = idist;
         a.idist2 = idist2;
         a.odist = odist;
         a.batch = batch;
         a.flags = flags;
         a.norm = norm;
-        HIP_TRY(launch_rows(e.logn, e.direction, kind, small ? e.row_var_small : e.row_var, a, st));
+        HIP_TRY(launch_rows(e.logn, e.direction, kind, e.row_var, a, st));
         return 0;
     }
     // Generic path: packed scratch of batch*n points each.

@@ -115,35 +115,23 @@ MUFFT_HD constexpr int rv_lds_per(int N, int var) {
 // per thread up to 8192 (c2c 1024 and 8192: -7%, -3% against 32).  At 16384
 // all kinds run 32 points per thread with the split exchange and registers
 // budgeted for two workgroups per CU (c2c -20%, r2c -7%; c2r with chunked
-// loads, -15%).  At 32768 all kinds run one workgroup of 32 points per
-// thread per transform (c2r with chunked loads); the two-workgroup split
-// reads its input twice and is 20% slower at 2^27 points, faster only for
-// small launches (rv_default_small).  65536 runs on two workgroups per
-// transform.
+// loads, -15%).  At 32768 c2c runs one workgroup of 32 points per thread
+// per transform, c2r too with mirrored lanes (65.6 against 74.2 us for the
+// chunked loads at 512 transforms, 509 against 580 at 4096); r2c splits each
+// transform over two workgroups budgeted to share a CU (60 against 75 us at
+// 512 transforms, 523 against 531 at 4096; the split reads its input twice,
+// from the L2 the second time).  65536 runs on two workgroups per transform.
 MUFFT_HD constexpr int rv_default_base(int N, int kind) {
     return N < 512 ? (kind == 3 && N == 256 ? rv_make(8, 0, 0) : rv_make(16, 0, 0))
          : N == 512 ? (kind == 3 ? rv_make(8, 0, 0) : rv_make(16, 0, 1))
          : N < 16384 ? (kind == 3 ? rv_with_mirror_lanes(rv_make(16, 0, 1)) : rv_make(16, 0, 1))
          : N == 16384 ? (kind == 3 ? rv_with_chunked_loads(rv_with_occ(rv_make(32, 1, 1), 2))
                                    : rv_with_occ(rv_make(32, 1, 1), 2))
-         : N == 32768 ? (kind == 3 ? rv_with_chunked_loads(rv_make(32, 1, 1)) : rv_make(32, 1, 1))
+         : N == 32768 ? (kind == 2 ? rv_with_occ(rv_make_split(1, rv_make(32, 1, 1)), 2)
+                                   : kind == 3 ? rv_with_mirror_lanes(rv_make(32, 1, 1)) : rv_make(32, 1, 1))
          : rv_make_split(1, rv_make(32, 1, 1));
 }
 MUFFT_HD constexpr int rv_default(int N, int kind = 0) { return rv_default_base(N, kind); }
-// Launches of few transforms run another variant where that measured faster
-// (0: the static choice for every batch).  The 32768-point real transforms
-// take the two-workgroup split (c2r with mirrored lanes) when a launch has at
-// most two transforms per CU; it doubles the workgroups of such a launch.
-// Cold on MI355X, r2c 65536 real points: 22-25 against 28 us at 64
-// transforms, 35 against 40 at 256, 65 against 75 at 512, but 127 against
-// 109 at 768 and 172 against 142 at 1024 (its workgroups both read the whole
-// input, and the second read misses once a launch outgrows the caches); c2r:
-// 26 against 27 us at 64, 72 against 76 at 512, 163 against 146 at 1024.
-MUFFT_HD constexpr int rv_default_small(int N, int kind) {
-    return N != 32768 || kind < 2 ? 0
-         : kind == 2 ? rv_with_occ(rv_make_split(1, rv_make(32, 1, 1)), 2)
-                     : rv_with_mirror_lanes(rv_with_occ(rv_make_split(1, rv_make(32, 1, 1)), 2));
-}
 
 // Radix sequence of NP passes: NP - 1 passes of radix R and one of the
 // remainder N / R^(NP-1), placed first (REM_FIRST) or last.  The twiddle

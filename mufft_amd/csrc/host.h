@@ -152,10 +152,6 @@ struct Engine {
     int n = 0, logn = 0, direction = 0;
     int row_var = 0;              // fused row-kernel variant
     float2 *tw_row = nullptr;     // sequence of the row variant
-    // Launches of at most small_max transforms run row_var_small instead
-    // (rv_default_small; only while no variant is chosen for the size).
-    int row_var_small = 0, small_max = 0;
-    float2 *tw_row_small = nullptr, *tw_split_small = nullptr;
     float2 *tw_col = nullptr;     // sequence of the column kernel
     float2 *tw_gen = nullptr;     // sequence of the generic pass kernels
     float2 *tw_split = nullptr;   // split variants: w_n^(dir*j*k1), 0 < k1 < N1, j < n/N1

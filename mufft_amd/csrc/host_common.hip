@@ -228,28 +228,10 @@ int Engine::init(int n_, int dir, int kind, bool rows, bool cols, hipStream_t s,
     logn = ilog2(n_);
     direction = dir;
     if (rows && row_fused()) {
-        const int tk = tune_kind(kind, dir);
-        row_var = var ? var : chosen_variant(logn, tk);
+        row_var = var ? var : chosen_variant(logn, tune_kind(kind, dir));
         const int n2 = rv_n(n, row_var);
         if (build_stockham_table(n2, dir, fused_sequence(n2, rv_R(n, row_var), true), &tw_row, s)) return -1;
         if (rv_split(row_var) > 0 && build_split_table(n, n2, dir, &tw_split, s)) return -1;
-        // The static choice may differ for small launches (rv_default_small):
-        // at most two transforms per CU.
-        const int vs = (var || variant_is_set(logn, tk)) ? 0 : rv_default_small(n, tk);
-        int dev = 0, cus = 0;
-        if (vs && vs != row_var &&
-            (hipGetDevice(&dev) != hipSuccess ||
-             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)) {
-            (void)hipGetLastError();   // no CU count: the static choice for every batch
-            cus = 0;
-        }
-        if (vs && vs != row_var && cus > 0) {
-            const int n2s = rv_n(n, vs);
-            if (build_stockham_table(n2s, dir, fused_sequence(n2s, rv_R(n, vs), true), &tw_row_small, s)) return -1;
-            if (rv_split(vs) > 0 && build_split_table(n, n2s, dir, &tw_split_small, s)) return -1;
-            row_var_small = vs;
-            small_max = 2 * cus;
-        }
     }
     if (cols && col_fused()) {
         col_tx = tx ? tx : chosen_col_tx(logn);
@@ -293,10 +275,9 @@ int Engine::init(int n_, int dir, int kind, bool rows, bool cols, hipStream_t s,
 
 void Engine::release() {
     for (float2 *t : {tw_row, tw_col, tw_gen, tw_split, tw_fs1, tw_fs2, tw_coarse, tw_fine, tw_pa, tw_pb,
-                      tw_coarse2, tw_fine2, tw_row_small, tw_split_small})
+                      tw_coarse2, tw_fine2})
         if (t) (void)hipFree(t);
-    tw_row = tw_col = tw_gen = tw_split = tw_row_small = tw_split_small = nullptr;
-    row_var_small = small_max = 0;
+    tw_row = tw_col = tw_gen = tw_split = nullptr;
     tw_fs1 = tw_fs2 = tw_coarse = tw_fine = nullptr;
     tw_pa = tw_pb = tw_coarse2 = tw_fine2 = nullptr;
 }

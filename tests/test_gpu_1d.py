@@ -192,9 +192,8 @@ def test_roundtrip_batched_4096(gpu):
 
 @pytest.mark.parametrize("kind", ["r2c", "c2r"])
 def test_65536_real_small_and_large_launches(gpu, kind):
-    """While no variant is chosen, a 65536-point real plan runs its
-    two-workgroup split for launches of at most two transforms per CU and one
-    workgroup per transform above (rv_default_small).  Both launches against
+    """A 65536-point real plan on a launch of 3 transforms and on one of more
+    than two transforms per CU (several rounds of workgroups): both against
     FP64 on the same leading transforms (test.c:58 bound), and the two results
     within rounding of each other."""
     import mufft_amd as M

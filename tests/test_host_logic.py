@@ -50,7 +50,8 @@ def test_defaults_follow_the_measured_policy(lib):
     # 16 points per thread up to 8192 complex points (c2r: 8 below 1024, mirrored
     # lanes from 1024); 16384 with registers for two workgroups per CU (c2r with
     # chunked loads); 32768 complex points in one workgroup of 32 points per
-    # thread (c2r with chunked loads); 65536 split over two workgroups.
+    # thread (c2r with mirrored lanes), r2c split over two workgroups budgeted
+    # to share a CU; 65536 split over two workgroups.
     M = 1 << 29
     assert lib.mufft_hip_default_variant(1024, 0) & 0xFF == 16
     assert lib.mufft_hip_default_variant(8192, 0) & 0xFF == 16
@@ -62,10 +63,10 @@ def test_defaults_follow_the_measured_policy(lib):
     assert (lib.mufft_hip_default_variant(16384, 0) >> 20) & 0xF == 2
     assert (lib.mufft_hip_default_variant(32768, 3) >> 20) & 0xF == 2
     assert (lib.mufft_hip_default_variant(32768, 0) >> 16) & 0xF == 0
-    assert lib.mufft_hip_default_variant(65536, 2) & 0xFF == 32
-    assert (lib.mufft_hip_default_variant(65536, 2) >> 16) & 0xF == 0
+    v = lib.mufft_hip_default_variant(65536, 2)
+    assert v & 0xFF == 32 and (v >> 16) & 0xF == 1 and (v >> 20) & 0xF == 2
     v = lib.mufft_hip_default_variant(65536, 3)
-    assert (v >> 16) & 0xF == 0 and v & (1 << 28)
+    assert (v >> 16) & 0xF == 0 and v & M
     assert (lib.mufft_hip_default_variant(65536, 0) >> 16) & 0xF == 1
 
 
