@@ -37,8 +37,9 @@ int cb_jobs_per_task(int logn) {
 
 int cb_points_per_thread(int logn) { return rv_E(cb_sub_n(logn), cb_variant(cb_sub_n(logn))); }
 
-// Jobs [t.job0, t.job0 + t.njobs) of a task, one fused transform each.
-template <int KIND, int DIR, int LOGN>
+// Jobs [t.job0, t.job0 + t.njobs) of a task, one fused transform each; FL:
+// the task's flags if fixed at compile time (RF_DYNAMIC: t.flags).
+template <int KIND, int DIR, int LOGN, unsigned FL>
 __device__ __forceinline__ void cb_run_rows(const CBTask &t, const CBJob *__restrict__ jobs, int tid,
                                             float2 *smem) {
     constexpr int N = 1 << LOGN;
@@ -64,12 +65,12 @@ __device__ __forceinline__ void cb_run_rows(const CBTask &t, const CBJob *__rest
         Y = j.spec2;
         Xo = j.out;
     }
-    row_transform<N, DIR, KIND, VAR>(X, Y, Xo, active, lt, t.tw, t.rtw, t.flags, t.norm,
+    row_transform<N, DIR, KIND, VAR, FL>(X, Y, Xo, active, lt, t.tw, t.rtw, t.flags, t.norm,
                                       smem + sub * LDSN, reinterpret_cast<float *>(smem) + sub * LDSN);
 }
 
 // Half t.k1 of one job's transform over two workgroups (split_impl.h).
-template <int KIND, int DIR, int LOGN>
+template <int KIND, int DIR, int LOGN, unsigned FL>
 __device__ __forceinline__ void cb_run_split(const CBTask &t, const CBJob *__restrict__ jobs, int tid,
                                              float2 *smem) {
     constexpr int N2 = 1 << (LOGN - 1);
@@ -79,15 +80,34 @@ __device__ __forceinline__ void cb_run_split(const CBTask &t, const CBJob *__res
     const float2 *X = t.type == CB_FWD_FIRST ? j.first : (t.type == CB_FWD_SECOND ? j.second : j.spec1);
     const float2 *Y = t.type == CB_INV ? j.spec2 : nullptr;
     float2 *Xo = t.type == CB_FWD_FIRST ? j.spec1 : (t.type == CB_FWD_SECOND ? j.spec2 : j.out);
-    split_transform<N2, 2, DIR, KIND, VAR>(X, Y, Xo, t.k1, tid, t.tw, t.stw, t.rtw, t.flags, t.norm, smem,
+    split_transform<N2, 2, DIR, KIND, VAR, FL>(X, Y, Xo, t.k1, tid, t.tw, t.stw, t.rtw, t.flags, t.norm, smem,
                                            reinterpret_cast<float *>(smem));
 }
 
+template <int KIND, int DIR, int LOGN, unsigned FL>
+__device__ __forceinline__ void cb_run_fl(const CBTask &t, const CBJob *__restrict__ jobs, int tid,
+                                          float2 *smem) {
+    if constexpr (LOGN >= CB_SPLIT_LOG2) cb_run_split<KIND, DIR, LOGN, FL>(t, jobs, tid, smem);
+    else cb_run_rows<KIND, DIR, LOGN, FL>(t, jobs, tid, smem);
+}
+
+// The flag sets of the zero-padded convolutions (both inputs padded, the
+// usual block convolution) run with their flags fixed at compile time; any
+// other set reads them at run time.  Mono: r2c, r2c, c2r with the multiply;
+// stereo: c2c, full r2c, c2c with the multiply (conv_batch_host.hip).
 template <int KIND, int DIR, int LOGN>
 __device__ __noinline__ void cb_run(const CBTask &t, const CBJob *__restrict__ jobs, int tid,
-                                       float2 *smem) {
-    if constexpr (LOGN >= CB_SPLIT_LOG2) cb_run_split<KIND, DIR, LOGN>(t, jobs, tid, smem);
-    else cb_run_rows<KIND, DIR, LOGN>(t, jobs, tid, smem);
+                                    float2 *smem) {
+    constexpr unsigned FWD = RF_ZERO_PAD | RF_KEEP_OUT;
+    if constexpr (KIND == KIND_R2C) {
+        if (t.flags == FWD) return cb_run_fl<KIND, DIR, LOGN, FWD>(t, jobs, tid, smem);
+        if (t.flags == (FWD | RF_FULL)) return cb_run_fl<KIND, DIR, LOGN, FWD | RF_FULL>(t, jobs, tid, smem);
+    } else if constexpr (KIND == KIND_C2C && DIR < 0) {
+        if (t.flags == FWD) return cb_run_fl<KIND, DIR, LOGN, FWD>(t, jobs, tid, smem);
+    } else {
+        if (t.flags == RF_MUL) return cb_run_fl<KIND, DIR, LOGN, (unsigned)RF_MUL>(t, jobs, tid, smem);
+    }
+    cb_run_fl<KIND, DIR, LOGN, RF_DYNAMIC>(t, jobs, tid, smem);
 }
 
 template <int KIND, int DIR, int LO, int HI>

@@ -33,6 +33,9 @@ enum RowFlag : unsigned {
 // Every flag: a launch with none of them runs the kernels' plain
 // instantiation, where the flags are compile-time zero.
 constexpr unsigned RF_ALL = (1u << 7) - 1u;
+// Template argument of the fused transforms: flags read at run time (any
+// other value fixes them at compile time).
+constexpr unsigned RF_DYNAMIC = ~0u;
 
 MUFFT_HD constexpr int ilog2(long long n) { return n <= 1 ? 0 : 1 + ilog2(n >> 1); }
 

@@ -24,11 +24,11 @@ namespace mufft {
 // transforms, so the pointers are wave-uniform and the strided traffic goes
 // through buffer descriptors (one VGPR offset per thread, scalar offsets per
 // slot).
-// PLAIN: the caller has checked that no runtime flag is set; they then fold
-// to constants, so no branch on them splits the load or store sequences
-// (such branches kept loads from issuing ahead of their use and raised
-// register pressure).
-template <int N, int DIR, int KIND, int VAR, bool PLAIN = false>
+// FIXED: the flags, when the caller knows them at compile time (RF_DYNAMIC:
+// read at run time).  Fixed flags fold to constants, so no branch on them
+// splits the load or store sequences (such branches kept loads from issuing
+// ahead of their use and raised register pressure).
+template <int N, int DIR, int KIND, int VAR, unsigned FIXED = RF_DYNAMIC>
 __device__ __forceinline__ void row_transform(const float2 *X, const float2 *Y, float2 *Xo,
                                               bool active, int lt, const float2 *tw,
                                               const float2 *rtw, unsigned flags, float norm,
@@ -48,7 +48,7 @@ __device__ __forceinline__ void row_transform(const float2 *X, const float2 *Y, 
     constexpr bool MIRROR = KIND == KIND_C2R && BUF && rv_mirror_lanes(VAR);
     auto slot = [](int o) { return XM == XM_FLAT ? o : pad_slot<LOGE>(o); };
 
-    if constexpr (PLAIN) flags = 0;
+    if constexpr (FIXED != RF_DYNAMIC) flags = FIXED;
     const bool zp = (flags & RF_ZERO_PAD) != 0;
     const bool mul = (flags & RF_MUL) != 0;
     const bool cj = (flags & RF_CONJ2) != 0;
@@ -283,7 +283,7 @@ __global__ void __launch_bounds__(rv_WG(N, VAR), rv_waves_per_simd(N, VAR)) fft_
     const int sub = tid / T;
     const long long g = (long long)blockIdx.x * B + sub;
     if ((a.flags & RF_ALL) == 0)
-        row_transform<N, DIR, KIND, VAR, true>(a.in + g * a.idist, a.in2 + g * a.idist2, a.out + g * a.odist,
+        row_transform<N, DIR, KIND, VAR, 0u>(a.in + g * a.idist, a.in2 + g * a.idist2, a.out + g * a.odist,
                                                g < a.batch, lt, a.tw, a.rtw, 0u, a.norm, smem + sub * LDSN,
                                                reinterpret_cast<float *>(smem) + sub * LDSN);
     else

@@ -27,8 +27,8 @@ namespace mufft {
 // Workgroup k1 of one transform: X / Y / Xo are the transform's input, second
 // operand and output, L / Lf its LDS (rv_lds_per(N2, VAR) slots).  All T
 // threads of the workgroup call it.
-// PLAIN: no runtime flag is set (as in row_transform).
-template <int N2, int N1, int DIR, int KIND, int VAR, bool PLAIN = false>
+// FIXED: compile-time flags, or RF_DYNAMIC (as in row_transform).
+template <int N2, int N1, int DIR, int KIND, int VAR, unsigned FIXED = RF_DYNAMIC>
 __device__ __forceinline__ void split_transform(const float2 *X, const float2 *Y, float2 *Xo, int k1,
                                                 int lt, const float2 *tw, const float2 *stw,
                                                 const float2 *rtw, unsigned flags, float norm,
@@ -46,7 +46,7 @@ __device__ __forceinline__ void split_transform(const float2 *X, const float2 *Y
     auto slot = [](int o) { return XM == XM_FLAT ? o : pad_slot<LOGE>(o); };
     k1 = __builtin_amdgcn_readfirstlane(k1);   // uniform: it feeds scalar buffer offsets
 
-    if constexpr (PLAIN) flags = 0;
+    if constexpr (FIXED != RF_DYNAMIC) flags = FIXED;
     const bool zp = (flags & RF_ZERO_PAD) != 0;
     const bool mul = (flags & RF_MUL) != 0;
     const bool cj = (flags & RF_CONJ2) != 0;
@@ -266,7 +266,7 @@ __global__ void __launch_bounds__(rv_T(N2, VAR), rv_waves_per_simd(N2, VAR)) fft
     const long long t = g / N1;
     const int k1 = (int)(g % N1);
     if ((a.flags & RF_ALL) == 0)
-        split_transform<N2, N1, DIR, KIND, VAR, true>(a.in + t * a.idist, a.in2 + t * a.idist2, a.out + t * a.odist,
+        split_transform<N2, N1, DIR, KIND, VAR, 0u>(a.in + t * a.idist, a.in2 + t * a.idist2, a.out + t * a.odist,
                                                       k1, threadIdx.x, a.tw, a.stw, a.rtw, 0u, a.norm, smem,
                                                       reinterpret_cast<float *>(smem));
     else
