@@ -282,8 +282,10 @@ __global__ void __launch_bounds__(rv_WG(N, VAR), rv_waves_per_simd(N, VAR)) fft_
     const int lt = tid & (T - 1);
     const int sub = tid / T;
     const long long g = (long long)blockIdx.x * B + sub;
-    if ((a.flags & RF_ALL) == 0)
-        row_transform<N, DIR, KIND, VAR, 0u>(a.in + g * a.idist, a.in2 + g * a.idist2, a.out + g * a.odist,
+    // r2c stores cacheable anyway: RF_KEEP_OUT changes nothing there.
+    constexpr unsigned MOOT = KIND == KIND_R2C ? (unsigned)RF_KEEP_OUT : 0u;
+    if ((a.flags & RF_ALL & ~MOOT) == 0)
+        row_transform<N, DIR, KIND, VAR, MOOT>(a.in + g * a.idist, a.in2 + g * a.idist2, a.out + g * a.odist,
                                                g < a.batch, lt, a.tw, a.rtw, 0u, a.norm, smem + sub * LDSN,
                                                reinterpret_cast<float *>(smem) + sub * LDSN);
     else

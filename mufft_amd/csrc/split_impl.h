@@ -265,7 +265,8 @@ __global__ void __launch_bounds__(rv_T(N2, VAR), rv_waves_per_simd(N2, VAR)) fft
     const unsigned g = xcd_block(blockIdx.x, gridDim.x);
     const long long t = g / N1;
     const int k1 = (int)(g % N1);
-    if ((a.flags & RF_ALL) == 0)
+    // The split's stores are cacheable anyway: RF_KEEP_OUT changes nothing.
+    if ((a.flags & RF_ALL & ~(unsigned)RF_KEEP_OUT) == 0)
         split_transform<N2, N1, DIR, KIND, VAR, 0u>(a.in + t * a.idist, a.in2 + t * a.idist2, a.out + t * a.odist,
                                                       k1, threadIdx.x, a.tw, a.stw, a.rtw, 0u, a.norm, smem,
                                                       reinterpret_cast<float *>(smem));
