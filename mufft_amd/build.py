@@ -17,6 +17,7 @@ LIB = PKG / "libmufft_hip.so"
 # Kernel units first (the row-kernel instantiations take the longest), then
 # the host units (planner, executor and C ABI).
 SOURCES = [
+    "rows_c2c_fwd_mid.hip", "rows_c2c_inv_mid.hip", "rows_c2r_mid.hip", "rows_r2c_mid.hip",
     "rows_c2r_large.hip", "rows_r2c_large.hip", "rows_c2c_fwd_large.hip", "rows_c2c_inv_large.hip",
     "rows_c2c_fwd.hip", "rows_c2c_inv.hip", "rows_r2c.hip", "rows_c2r.hip", "conv_batch.hip",
     "cols.hip", "misc.hip",

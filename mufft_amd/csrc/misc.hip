@@ -7,12 +7,17 @@
 
 namespace mufft {
 
-// Each fills its (direction, kind) table: entries [0, kRowSplitIndex), and
-// the _large ones [kRowSplitIndex, kNumRowVariants).
+// Each fills its (direction, kind) table: entries [0, kRowMidIndex), the
+// _mid ones [kRowMidIndex, kRowSplitIndex), the _large ones
+// [kRowSplitIndex, kNumRowVariants).
 void row_kernels_c2c_fwd(const void **out);
 void row_kernels_c2c_inv(const void **out);
 void row_kernels_r2c(const void **out);
 void row_kernels_c2r(const void **out);
+void row_kernels_c2c_fwd_mid(const void **out);
+void row_kernels_c2c_inv_mid(const void **out);
+void row_kernels_r2c_mid(const void **out);
+void row_kernels_c2r_mid(const void **out);
 void row_kernels_c2c_fwd_large(const void **out);
 void row_kernels_c2c_inv_large(const void **out);
 void row_kernels_r2c_large(const void **out);
@@ -26,6 +31,10 @@ struct RowTables {
         row_kernels_c2c_inv(t[1]);
         row_kernels_r2c(t[2]);
         row_kernels_c2r(t[3]);
+        row_kernels_c2c_fwd_mid(t[0]);
+        row_kernels_c2c_inv_mid(t[1]);
+        row_kernels_r2c_mid(t[2]);
+        row_kernels_c2r_mid(t[3]);
         row_kernels_c2c_fwd_large(t[0]);
         row_kernels_c2c_inv_large(t[1]);
         row_kernels_r2c_large(t[2]);

@@ -81,11 +81,13 @@ constexpr RowVariantEntry kRowVariants[] = {
 };
 constexpr int kNumRowVariants = sizeof(kRowVariants) / sizeof(kRowVariants[0]);
 
-// The instantiation units split the registry here (the entries are ordered
-// by size; from 32768 points on they take most of the compile time).
-constexpr int row_split_index(int i = 0) {
-    return i >= kNumRowVariants || kRowVariants[i].logN >= 15 ? i : row_split_index(i + 1);
+// The instantiation units split the registry at the first entry of 8192 and
+// of 32768 points (the entries are ordered by size; the larger sizes take
+// most of the compile time).
+constexpr int row_split_index(int logn, int i = 0) {
+    return i >= kNumRowVariants || kRowVariants[i].logN >= logn ? i : row_split_index(logn, i + 1);
 }
-constexpr int kRowSplitIndex = row_split_index();
+constexpr int kRowMidIndex = row_split_index(13);
+constexpr int kRowSplitIndex = row_split_index(15);
 
 }  // namespace mufft
