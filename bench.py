@@ -71,9 +71,9 @@ def host_cpu_info():
 
 def baseline_threads() -> int:
     """Threads for the multi-core CPU baseline: the CPUs this process may use,
-    capped by OMP_NUM_THREADS when set.  A GPU box grants one GPU's job a
-    share of its host (OMP_NUM_THREADS=16 there), so the baseline uses that
-    share rather than every core of the machine."""
+    capped by OMP_NUM_THREADS when set.  A shared GPU host may grant one
+    GPU's job only a share of its cores (and set OMP_NUM_THREADS to it), so
+    the baseline uses that share rather than every core of the machine."""
     n = host_cpu_info()["usable_cpus"]
     env = os.environ.get("OMP_NUM_THREADS")
     if env and env.isdigit() and int(env) > 0:
@@ -608,7 +608,7 @@ def main():
 
     # candidates: the tuned variant, the static default and one workgroup of
     # 32 points per thread (V32ST; for c2r with chunked loads), which won the
-    # step on some boxes
+    # step on some MI355X machines
     one_wg_r, one_wg_c = 0x1120, 0x10001120
     def_r, def_c = L.mufft_hip_default_variant(REAL_N, 2), L.mufft_hip_default_variant(REAL_N, 3)
     cand_r = sorted({int(tuned[f"r2c_{REAL_N}"]["variant"], 16), def_r, one_wg_r})
