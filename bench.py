@@ -483,17 +483,108 @@ def diff_conv_bench(dev, stream, steps: int = 10, warmup: int = 3):
     }
 
 
+# The per-path throughput table: (name, kind, N or image side, batch); 2D
+# entries carry "2d_" kinds.  Every launch moves 256 MiB to 1 GiB.
+PATHS = [
+    ("c2c 4096", "c2c", 4096, 16384), ("r2c 65536", "r2c", 65536, 512), ("c2r 65536", "c2r", 65536, 512),
+    ("c2c 1024", "c2c", 1024, 32768), ("c2c 8192", "c2c", 8192, 4096), ("c2c 16384", "c2c", 16384, 2048),
+    ("c2c 32768", "c2c", 32768, 1024), ("c2c 65536", "c2c", 65536, 512),
+    ("2D c2c 1024^2", "2d_c2c", 1024, 16), ("2D r2c 2048^2", "2d_r2c", 2048, 4), ("2D c2r 2048^2", "2d_c2r", 2048, 4),
+    ("2D c2c 4096^2", "2d_c2c", 4096, 2), ("2D c2c 8192^2", "2d_c2c", 8192, 1),
+    ("four-step 2^17", "c2c", 1 << 17, 512), ("four-step 2^20", "c2c", 1 << 20, 64),
+    ("four-step 2^22", "c2c", 1 << 22, 16), ("four-step 2^23", "c2c", 1 << 23, 8),
+    ("four-step 2^24", "c2c", 1 << 24, 2), ("four-step 2^26", "c2c", 1 << 26, 1),
+]
+
+
+def paths_bench(reps: int = 12):
+    """Throughput of every planner path on inputs resident in HBM, with cold
+    caches: a 512 MiB streaming copy (which evicts the 256 MB Infinity Cache)
+    precedes each launch, and the launch alone is timed with HIP events on
+    its stream; the median of `reps` launches.  Algorithmic bytes: one read
+    and one write of every point (c2c 16 B, r2c / c2r 4 + 8 B per real
+    sample pair as in the main line); 2D counts one read and one write of each
+    image.  Prints one JSON line."""
+    import torch
+
+    import mufft_amd as M
+    from mufft_amd import _lib
+
+    L = _lib.load()
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(0x5001)
+    f0 = torch.empty(64 << 20, dtype=torch.complex64, device=dev)
+    f1 = torch.empty_like(f0)
+
+    def cold_us(fn):
+        ts = []
+        for i in range(reps + 2):
+            assert L.mufft_hip_stream_copy(f1.data_ptr(), f0.data_ptr(), f0.numel() * 8, None) == 0
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize()
+            if i >= 2:
+                ts.append(e0.elapsed_time(e1) * 1e3)
+        ts.sort()
+        return ts[len(ts) // 2]
+
+    rows = []
+    for name, kind, n, b in PATHS:
+        if kind.startswith("2d_"):
+            k = kind[3:]
+            if k == "r2c":
+                x = torch.rand((b, n, n), generator=g, device=dev)
+                y = torch.empty((b, n, n), dtype=torch.complex64, device=dev)
+            elif k == "c2r":
+                x = torch.randn((b, n, n), generator=g, device=dev, dtype=torch.complex64)
+                y = torch.empty((b, n, n), device=dev)
+            else:
+                x = torch.randn((b, n, n), generator=g, device=dev, dtype=torch.complex64)
+                y = torch.empty_like(x)
+            p = M.Plan2D(k, n, n, M.INVERSE if k == "c2r" else M.FORWARD)
+            alg = b * n * (16.0 * n if k == "c2c" else 4.0 * n + 8.0 * (n // 2 + 1))
+        else:
+            if kind == "r2c":
+                x = torch.rand((b, n), generator=g, device=dev)
+                y = torch.empty((b, n // 2 + 1), dtype=torch.complex64, device=dev)
+            elif kind == "c2r":
+                x = torch.randn((b, n // 2 + 1), generator=g, device=dev, dtype=torch.complex64)
+                y = torch.empty((b, n), device=dev)
+            else:
+                x = torch.randn((b, n), generator=g, device=dev, dtype=torch.complex64)
+                y = torch.empty_like(x)
+            p = M.Plan1D(kind, n, M.INVERSE if kind == "c2r" else M.FORWARD)
+            alg = b * (16.0 * n if kind == "c2c" else 4.0 * n + 8.0 * (n // 2 + 1))
+        us = cold_us(lambda: p.execute_batched(y, x, b))
+        rows.append({"path": name, "batch": b, "us": round(us, 1), "TBs": round(alg / us / 1e6, 2),
+                     "hbm_frac": round(alg / us / 1e6 / (HBM_PEAK_GBS / 1e3), 3)})
+        print(f"{name} x {b}: {us:.1f} us, {alg / us / 1e6:.2f} TB/s", file=sys.stderr, flush=True)
+        p.close()
+        del x, y
+    print(json.dumps({"paths": rows, "method": "cold caches (512 MiB copy before each launch), median of "
+                      f"{reps} HIP-event-timed launches"}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--paths", action="store_true",
+                    help="time every planner path with cold caches (one JSON line) instead of the step")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the batch of independent "
               "transforms would shard across GPUs with no collective.", file=sys.stderr)
         sys.exit(2)
+    if args.paths:
+        paths_bench()
+        return
 
     import torch
 

@@ -60,12 +60,39 @@ __global__ void __launch_bounds__(col_T(L_) * (TXP ? TXP : col_TX(L_))) fft_cols
         });
     }
 
-    // Remainder pass last (seq_radix): the lanes of one column thread share its twiddles.
+    // COL_TWIDDLE: the twiddle of point t + T*c of a column with factor b is
+    // w^(b*t) * w^(b*T*c), a factor per thread and E steps per column in LDS
+    // (as in the four-step column kernel below), where a workgroup has a
+    // thread per step.
+    constexpr bool STEPS = MODE == COL_TWIDDLE && E <= T;
+    __shared__ float2 steps[STEPS ? E * TX : 1];
+    float2 w0 = make_float2(1.f, 0.f);
+    if constexpr (STEPS) {
+        const unsigned smask = (1u << a.log2s) - 1u;
+        const int scol = tile * TX + tid % TX;
+        if (tid < E * TX) {
+            const unsigned es = ((unsigned)scol >> a.log2div) * (unsigned)(T * (tid / TX));
+            steps[tid] = scol < a.ncols ? cmul(a.coarse[es >> a.log2s], a.fine[es & smask]) : make_float2(1.f, 0.f);
+        }
+        if (active) {
+            const unsigned e0 = ((unsigned)col >> a.log2div) * (unsigned)t;
+            w0 = cmul(a.coarse[e0 >> a.log2s], a.fine[e0 & smask]);
+        }
+    }
+
+    // Remainder pass last (seq_radix): the lanes of one column thread share its
+    // twiddles.  Its exchanges synchronise the workgroup (the steps are visible
+    // after them).
     stockham_passes<L_, E, DIR, XM, col_twm(L_), E, false, false>(v, t, TwGlobal{a.tw}, smem,
                                                                   reinterpret_cast<float *>(smem), slot);
 
     if (active) {
-        if constexpr (MODE == COL_TWIDDLE) {
+        if constexpr (STEPS) {
+            static_for<0, E>([&](auto C) {
+                constexpr int c = decltype(C)::value;
+                v[c] = cmul(v[c], cmul(w0, steps[c * TX + x]));
+            });
+        } else if constexpr (MODE == COL_TWIDDLE) {
             const unsigned b = (unsigned)col >> a.log2div;
             const unsigned smask = (1u << a.log2s) - 1u;
             static_for<0, E>([&](auto C) {
@@ -126,13 +153,38 @@ __global__ void __launch_bounds__(fs_WG(L_)) fft_four_step_cols_kernel(FourStepA
         v[c] = (zp && row >= L_ / 2) ? make_float2(0.f, 0.f) : src[(long long)row * a.n2];
     });
 
+    // The twiddle of point (n2, k1) = (col, t + T*c) is w^(col*t) * w^(col*T*c):
+    // a factor per thread and E steps per column, the steps of the tile's
+    // columns in LDS, one coarse*fine product each (every product is within
+    // about 1.5 ulp; a point's twiddle within about 3.5).  Two table gathers
+    // per point instead cost 10-16% of the whole transform.  Used where a
+    // workgroup has a thread per step (E <= T, columns from 256 points).
+    constexpr bool STEPS = E <= T;
+    __shared__ float2 steps[STEPS ? E * TX : 1];
+    const int smask = (1 << a.log2s) - 1;
+    float2 w0 = make_float2(1.f, 0.f);
+    if constexpr (STEPS) {
+        if (tid < E * TX) {
+            const unsigned es = (unsigned)(tile * TX + tid % TX) * (unsigned)(T * (tid / TX));   // < N
+            steps[tid] = cmul(a.coarse[es >> a.log2s], a.fine[es & smask]);
+        }
+        const unsigned e0 = (unsigned)col * (unsigned)t;
+        w0 = cmul(a.coarse[e0 >> a.log2s], a.fine[e0 & smask]);
+    }
+
+    // The passes' exchanges synchronise the workgroup: the steps are
+    // visible to every thread after them.
     stockham_passes<L_, E, DIR, XM_PAD, TW_FULL, E, false, false>(v, t, TwGlobal{a.tw}, smem, nullptr, slot);
 
-    const int smask = (1 << a.log2s) - 1;
     static_for<0, E>([&](auto C) {
         constexpr int c = decltype(C)::value;
-        const unsigned e = (unsigned)col * (unsigned)(t + T * c);   // < N
-        const float2 w = cmul(a.coarse[e >> a.log2s], a.fine[e & smask]);
+        float2 w;
+        if constexpr (STEPS) {
+            w = cmul(w0, steps[c * TX + x]);
+        } else {
+            const unsigned e = (unsigned)col * (unsigned)(t + T * c);   // < N
+            w = cmul(a.coarse[e >> a.log2s], a.fine[e & smask]);
+        }
         smem[x * LP + t + T * c] = cmul(v[c], w);
     });
     __syncthreads();
