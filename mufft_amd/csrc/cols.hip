@@ -173,8 +173,9 @@ __global__ void __launch_bounds__(fs_WG(L_)) fft_four_step_cols_kernel(FourStepA
     }
 
     // The passes' exchanges synchronise the workgroup: the steps are
-    // visible to every thread after them.
-    stockham_passes<L_, E, DIR, XM_PAD, TW_FULL, E, false, false>(v, t, TwGlobal{a.tw}, smem, nullptr, slot);
+    // visible to every thread after them.  Split pass twiddles from 1024
+    // points on, as in the column kernel (2^20, 2^22: -2.5%).
+    stockham_passes<L_, E, DIR, XM_PAD, col_twm(L_), E, false, false>(v, t, TwGlobal{a.tw}, smem, nullptr, slot);
 
     static_for<0, E>([&](auto C) {
         constexpr int c = decltype(C)::value;
