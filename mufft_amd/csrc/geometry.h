@@ -201,13 +201,14 @@ constexpr int COL_MAX_LOG2 = 13;   // fused column kernel covers L <= 8192
 // kernel on N1-point columns, then A-point and B-point column passes (at most
 // 1024 points, tiles of 64-128-byte row segments) over N2 = A * B.  Returns
 // log2 N1 << 8 | log2 A << 4 | log2 B, or 0 where two launches are used.
-// From a sweep of the splits on MI355X at 512 MiB per launch: against the
-// best two-launch split, -8% at 2^24, -16% at 2^25, -25% at 2^26 (2^23 and
-// below: no gain).
+// From a sweep of every split on MI355X at 512 MiB per launch, with the
+// per-column twiddle steps: against the best two-launch split, -6% at 2^23,
+// -8% at 2^24, -16% at 2^25, -25% at 2^26 (2^22 and below: no gain).
 MUFFT_HD constexpr int fs3_split(int logn) {
-    return logn == 24 ? (10 << 8) | (8 << 4) | 6
+    return logn == 23 ? (9 << 8) | (8 << 4) | 6
+         : logn == 24 ? (10 << 8) | (8 << 4) | 6
          : logn == 25 ? (9 << 8) | (8 << 4) | 8
-         : logn == 26 ? (10 << 8) | (9 << 4) | 7
+         : logn == 26 ? (10 << 8) | (10 << 4) | 6
          : 0;
 }
 // Split codes of a large transform (the planner's static choice below, or

@@ -14,7 +14,7 @@ from tests.helpers import abs_tol, cplx, empty_dev, from_dev, real, rel_rms, to_
 
 pytestmark = pytest.mark.gpu
 
-THREE_PASS_FROM = 24   # first size (log2) the planner splits into three launches
+THREE_PASS_FROM = 23   # first size (log2) the planner splits into three launches
 
 
 def c2c_launches(logn):

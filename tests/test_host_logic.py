@@ -189,10 +189,12 @@ def _code(n1, a=0, b=0):
 
 
 def test_large_splits_static_choice_and_validation(lib):
-    # The static splits of the four-step sizes (a sweep on MI355X): 2^24..2^26
-    # in three launches, 2^17..2^23 in two.
+    # The static splits of the four-step sizes (a sweep on MI355X): 2^23..2^26
+    # in three launches, 2^17..2^22 in two.
+    assert lib.mufft_hip_get_large_split(1 << 23, 0) == _code(9, 8, 6)
     assert lib.mufft_hip_get_large_split(1 << 24, 0) == _code(10, 8, 6)
-    assert lib.mufft_hip_get_large_split(1 << 26, 1) == _code(10, 9, 7)
+    assert lib.mufft_hip_get_large_split(1 << 26, 1) == _code(10, 10, 6)
+    assert lib.mufft_hip_get_large_split(1 << 22, 0) == _code(12)
     assert lib.mufft_hip_get_large_split(1 << 17, 0) == _code(9)
     assert lib.mufft_hip_get_large_split(1 << 18, 2) == _code(9)   # r2c of 2^18 reals: 2^17 complex
     # fused sizes and sizes beyond the four-step range have no split
