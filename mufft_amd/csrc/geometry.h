@@ -166,9 +166,10 @@ MUFFT_HD constexpr int seq_first(int N, int R, int NP, bool rem_first) {
 MUFFT_HD constexpr int col_E(int L) { return L <= 16 ? L : (L <= 4096 ? 16 : 32); }
 MUFFT_HD constexpr int col_T(int L) { return L / col_E(L); }
 // Tile width of the four-step column kernel (whose transposed store needs the
-// full complex tile in LDS).
+// full complex tile in LDS).  Two columns from 4096 points on, so that two
+// tiles share a CU: 2^22 x 4 -7%, the same time at 2^22 x 16.
 MUFFT_HD constexpr int fs_TX(int L) {
-    return col_T(L) <= 32 ? 256 / col_T(L) : (col_T(L) <= 128 ? 8 : (L >= 8192 ? 2 : 4));
+    return col_T(L) <= 32 ? 256 / col_T(L) : (col_T(L) <= 128 ? 8 : (L >= 4096 ? 2 : 4));
 }
 MUFFT_HD constexpr int fs_WG(int L) { return col_T(L) * fs_TX(L); }
 // The plain column kernel, from sweeps on MI355X.  From L = 1024 on it uses
@@ -178,7 +179,7 @@ MUFFT_HD constexpr int fs_WG(int L) { return col_T(L) * fs_TX(L); }
 // 12% faster than 8; at L = 4096 the split exchange lets two 4-column tiles
 // share a CU (2D 4096^2: -5%); at L = 8192 it allows 4 columns instead of 2
 // (2D 8192^2: -6%, four-step 2^26: -19%).
-MUFFT_HD constexpr int col_TX(int L) { return L == 1024 ? 16 : (L == 2048 ? 4 : (L == 8192 ? 4 : fs_TX(L))); }
+MUFFT_HD constexpr int col_TX(int L) { return L == 1024 ? 16 : (L == 2048 || L == 4096 || L == 8192 ? 4 : fs_TX(L)); }
 // The one alternative tile width compiled per column length (0: none): twice
 // the default below 1024 points, half or twice it from 1024 on where the LDS
 // and the 1024-thread workgroup allow.  The tile width changes only which
