@@ -93,8 +93,8 @@ size_t plan_2d_scratch_points(const mufft_plan_2d *p) {
     return 0;
 }
 
-int exec_2d_with(mufft_plan_2d *p, void *out_, const void *in_, unsigned howmany, DevBuf &s0,
-                 DevBuf &s1, DevBuf &s2, DevBuf &alt, hipStream_t st) {
+static int exec_2d_images(mufft_plan_2d *p, void *out_, const void *in_, unsigned howmany, DevBuf &s0,
+                          DevBuf &s1, DevBuf &s2, DevBuf &alt, hipStream_t st) {
     const long long Nx = p->Nx, Ny = p->Ny;
     const int hm = (int)howmany;
     const float2 *in = static_cast<const float2 *>(in_);
@@ -123,6 +123,29 @@ int exec_2d_with(mufft_plan_2d *p, void *out_, const void *in_, unsigned howmany
         return -1;
     return run_rows(p->ex, KIND_C2R, p->rtw, s2.f2(), mid_rs, nullptr, 0, out, Nx, (int)(Ny * hm),
                     zp, 1.0f, s0, s1, st);
+}
+
+// A batch whose images hold more than this many bytes (the larger of input
+// and output) runs in balanced chunks of at most this size, each chunk's row
+// pass and column pass back to back: the rows' output then stays in the
+// 256 MB Infinity Cache for the column pass.  2D c2c 1024^2 x 64 (512 MiB):
+// 432 -> 359 us in chunks of 32 images, 381 us in chunks of 16.
+constexpr size_t k2dChunkBytes = size_t(256) << 20;
+
+int exec_2d_with(mufft_plan_2d *p, void *out, const void *in, unsigned howmany, DevBuf &s0,
+                 DevBuf &s1, DevBuf &s2, DevBuf &alt, hipStream_t st) {
+    const size_t img = p->in_bytes > p->out_bytes ? p->in_bytes : p->out_bytes;
+    const size_t per = img >= k2dChunkBytes ? 1 : k2dChunkBytes / img;
+    if (howmany <= per) return exec_2d_images(p, out, in, howmany, s0, s1, s2, alt, st);
+    const size_t nchunks = (howmany + per - 1) / per;
+    const unsigned ch = (unsigned)((howmany + nchunks - 1) / nchunks);
+    for (unsigned i = 0; i < howmany; i += ch) {
+        const unsigned n = howmany - i < ch ? howmany - i : ch;
+        if (exec_2d_images(p, static_cast<char *>(out) + (size_t)i * p->out_bytes,
+                           static_cast<const char *>(in) + (size_t)i * p->in_bytes, n, s0, s1, s2, alt, st))
+            return -1;
+    }
+    return 0;
 }
 
 // howmany packed images with the plan's own scratch.

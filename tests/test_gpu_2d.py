@@ -123,3 +123,45 @@ def test_2d_large_and_generic(gpu, Nx, Ny):
     Y = from_dev(dY)[:, : Nx // 2 + 1]
     assert rel_rms(Y, fp64.r2c_2d(xr)) < 2e-6
     assert rel_rms(from_dev(dz) / (Nx * Ny), xr) < 2e-6
+
+
+@pytest.mark.parametrize("kind,n,batch", [("c2c", 1024, 40), ("r2c", 2048, 9), ("c2r", 2048, 9)])
+def test_2d_batch_beyond_chunk_size(gpu, kind, n, batch):
+    """A batch of more than 256 MiB of images runs in chunks (each chunk's row
+    and column passes back to back): every image must equal the same image
+    transformed alone, bit for bit, and the first must match FP64."""
+    import torch
+
+    import mufft_amd as M
+    g = torch.Generator(device=gpu)
+    g.manual_seed(0x3301)
+    if kind == "c2c":
+        x = torch.randn((batch, n, n), dtype=torch.complex64, device=gpu, generator=g)
+        y = torch.empty_like(x)
+    elif kind == "r2c":
+        x = torch.rand((batch, n, n), device=gpu, generator=g)
+        y = torch.empty((batch, n, n), dtype=torch.complex64, device=gpu)
+    else:
+        # c2r input: the padded layout, Nx/2+1 meaningful bins per row of Nx.
+        x = torch.randn((batch, n, n), dtype=torch.complex64, device=gpu, generator=g)
+        y = torch.empty((batch, n, n), device=gpu)
+    p = M.Plan2D(kind, n, n, M.INVERSE if kind == "c2r" else M.FORWARD)
+    assert max(x[0].numel() * x.element_size(), y[0].numel() * y.element_size()) * batch > 256 << 20
+    p.execute_batched(y, x, batch)
+    one = torch.empty_like(y[0:1])
+    cols = n // 2 + 1 if kind == "r2c" else n   # r2c rows: Nx/2+1 bins of a padded row
+    for b in (0, batch // 2, batch - 1):
+        p.execute_batched(one, x[b:b + 1].clone(), 1)
+        assert torch.equal(one[0, :, :cols], y[b, :, :cols]), (kind, b)
+    x0 = x[0].cpu().numpy()
+    if kind == "c2c":
+        ref = np.fft.fft2(x0.astype(np.complex128))
+        got = y[0].cpu().numpy()
+    elif kind == "r2c":
+        ref = np.fft.rfft2(x0.astype(np.float64))
+        got = y[0, :, : n // 2 + 1].cpu().numpy()
+    else:
+        ref = fp64.c2r_2d(x0, n)   # columns first, then the rows' c2r (fft.c:863-916)
+        got = y[0].cpu().numpy()
+    assert rel_rms(got, ref) < 2e-6, kind
+    p.close()
