@@ -69,10 +69,9 @@ __global__ void __launch_bounds__(col_T(L_) * (TXP ? TXP : col_TX(L_))) fft_cols
     float2 w0 = make_float2(1.f, 0.f);
     if constexpr (STEPS) {
         const unsigned smask = (1u << a.log2s) - 1u;
-        const int scol = tile * TX + tid % TX;
-        if (tid < E * TX) {
-            const unsigned es = ((unsigned)scol >> a.log2div) * (unsigned)(T * (tid / TX));
-            steps[tid] = scol < a.ncols ? cmul(a.coarse[es >> a.log2s], a.fine[es & smask]) : make_float2(1.f, 0.f);
+        if (tid < E * TX) {   // step tid / TX of this thread's own column (tid % TX == x)
+            const unsigned es = ((unsigned)col >> a.log2div) * (unsigned)(T * (tid / TX));
+            steps[tid] = col < a.ncols ? cmul(a.coarse[es >> a.log2s], a.fine[es & smask]) : make_float2(1.f, 0.f);
         }
         if (active) {
             const unsigned e0 = ((unsigned)col >> a.log2div) * (unsigned)t;
