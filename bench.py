@@ -494,6 +494,7 @@ PATHS = [
     ("four-step 2^17", "c2c", 1 << 17, 512), ("four-step 2^20", "c2c", 1 << 20, 64),
     ("four-step 2^22", "c2c", 1 << 22, 16), ("four-step 2^23", "c2c", 1 << 23, 8),
     ("four-step 2^24", "c2c", 1 << 24, 2), ("four-step 2^26", "c2c", 1 << 26, 1),
+    ("r2c 2^21 (four-step + resolve)", "r2c", 1 << 21, 64), ("c2r 2^21 (resolve + four-step)", "c2r", 1 << 21, 64),
 ]
 
 
