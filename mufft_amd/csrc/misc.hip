@@ -156,13 +156,16 @@ hipError_t launch_pass(int radix, int direction, const PassArgs &a, hipStream_t 
 // Standalone real-transform resolves (kernel.c:36-82) for sizes beyond the
 // fused kernels, and the pointwise multiply (kernel.c:22-34).
 // ---------------------------------------------------------------------------
+// n is a power of two (the launchers check): the bin and transform of an
+// element come from a mask and a shift, not a 64-bit division.
 __global__ void __launch_bounds__(256) resolve_r2c_kernel(ResolveArgs a) {
     const long long total = a.n * a.batch;
     const bool full = (a.flags & RF_FULL) != 0;
+    const int lg = __builtin_ctzll((unsigned long long)a.n);
     for (long long gid = (long long)blockIdx.x * 256 + threadIdx.x; gid < total;
          gid += (long long)gridDim.x * 256) {
-        const long long k = gid % a.n;
-        const long long g = gid / a.n;
+        const long long k = gid & (a.n - 1);
+        const long long g = gid >> lg;
         const float2 *Z = a.in + g * a.idist;
         float2 *X = a.out + g * a.odist;
         const float2 za = Z[k];
@@ -186,10 +189,11 @@ __global__ void __launch_bounds__(256) resolve_c2r_kernel(ResolveArgs a) {
     const long long total = a.n * a.batch;
     const bool mul = (a.flags & RF_MUL) != 0;
     const bool cj = (a.flags & RF_CONJ2) != 0;
+    const int lg = __builtin_ctzll((unsigned long long)a.n);
     for (long long gid = (long long)blockIdx.x * 256 + threadIdx.x; gid < total;
          gid += (long long)gridDim.x * 256) {
-        const long long k = gid % a.n;
-        const long long g = gid / a.n;
+        const long long k = gid & (a.n - 1);
+        const long long g = gid >> lg;
         if ((a.flags & RF_ZERO_PAD) && k >= a.n / 2) {
             a.out[g * a.odist + k] = make_float2(0.f, 0.f);
             continue;
@@ -335,6 +339,7 @@ hipError_t launch_mul_reduce(const MulArgs &a, hipStream_t s) {
 hipError_t launch_resolve_r2c(const ResolveArgs &a, hipStream_t s) {
     const long long work = a.n * a.batch;
     if (work <= 0) return hipSuccess;
+    if ((a.n & (a.n - 1)) != 0) return hipErrorInvalidValue;
     hipLaunchKernelGGL(resolve_r2c_kernel, dim3(grid_for(work, 256)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
@@ -342,6 +347,7 @@ hipError_t launch_resolve_r2c(const ResolveArgs &a, hipStream_t s) {
 hipError_t launch_resolve_c2r(const ResolveArgs &a, hipStream_t s) {
     const long long work = a.n * a.batch;
     if (work <= 0) return hipSuccess;
+    if ((a.n & (a.n - 1)) != 0) return hipErrorInvalidValue;
     hipLaunchKernelGGL(resolve_c2r_kernel, dim3(grid_for(work, 256)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
