@@ -125,18 +125,22 @@ static int exec_2d_images(mufft_plan_2d *p, void *out_, const void *in_, unsigne
                     zp, 1.0f, s0, s1, st);
 }
 
-// A batch whose images hold more than this many bytes (the larger of input
-// and output) runs in balanced chunks of at most this size, each chunk's row
-// pass and column pass back to back: the rows' output then stays in the
-// 256 MB Infinity Cache for the column pass.  2D c2c 1024^2 x 64 (512 MiB):
-// 432 -> 359 us in chunks of 32 images, 381 us in chunks of 16.
+// A c2c batch of images of 8-32 MiB (1024^2 to 2048^2 points) that holds
+// more than this many bytes runs in balanced chunks of at most this size,
+// each chunk's row pass and column pass back to back: the rows' output then
+// stays in the 256 MB Infinity Cache for the column pass.  Measured on
+// MI355X, 512 MiB per batch: c2c 1024^2 x 64 445 -> 389 us, 2048^2 x 16
+// 583 -> 454 us; but c2c 512^2 x 256 +2%, r2c / c2r 2048^2 x 16 +6-7% and
+// r2c 1024^2 x 64 +4% (their padded rows leave half of the output buffer
+// untouched), so those run whole.
 constexpr size_t k2dChunkBytes = size_t(256) << 20;
 
 int exec_2d_with(mufft_plan_2d *p, void *out, const void *in, unsigned howmany, DevBuf &s0,
                  DevBuf &s1, DevBuf &s2, DevBuf &alt, hipStream_t st) {
-    const size_t img = p->in_bytes > p->out_bytes ? p->in_bytes : p->out_bytes;
-    const size_t per = img >= k2dChunkBytes ? 1 : k2dChunkBytes / img;
-    if (howmany <= per) return exec_2d_images(p, out, in, howmany, s0, s1, s2, alt, st);
+    const size_t img = p->in_bytes;
+    const size_t per = k2dChunkBytes / img;
+    if (p->kind != KIND_C2C || img < (size_t(8) << 20) || img > (size_t(32) << 20) || howmany <= per)
+        return exec_2d_images(p, out, in, howmany, s0, s1, s2, alt, st);
     const size_t nchunks = (howmany + per - 1) / per;
     const unsigned ch = (unsigned)((howmany + nchunks - 1) / nchunks);
     for (unsigned i = 0; i < howmany; i += ch) {

@@ -127,9 +127,9 @@ def test_2d_large_and_generic(gpu, Nx, Ny):
 
 @pytest.mark.parametrize("kind,n,batch", [("c2c", 1024, 40), ("r2c", 2048, 9), ("c2r", 2048, 9)])
 def test_2d_batch_beyond_chunk_size(gpu, kind, n, batch):
-    """A batch of more than 256 MiB of images runs in chunks (each chunk's row
-    and column passes back to back): every image must equal the same image
-    transformed alone, bit for bit, and the first must match FP64."""
+    """A batch of more than 256 MiB of images (c2c: run in chunks, each
+    chunk's row and column passes back to back): every image must equal the
+    same image transformed alone, bit for bit, and the first must match FP64."""
     import torch
 
     import mufft_amd as M
